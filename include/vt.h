@@ -1,0 +1,183 @@
+/*
+ * vt.h -- C-ABI of the MI355X voxel ray-traversal library (libvt.so).
+ *
+ * This is the drop-in boundary for the reference engine's render dispatch:
+ * what `Game.render` (src/game.zig:232-256) binds and dispatches through
+ * `ComputePipeline.dispatch` (src/engine/graphics/shader.zig:113-117) --
+ * camera UBO 8, block pool SSBO 9, chunk table SSBO 10, model atlas image 6,
+ * G-buffer images 0-3 -- becomes a world handle, a camera struct and caller
+ * owned device output buffers, traced by hand-written gfx950 HIP kernels.
+ *
+ * Conventions
+ *  - Every entry point returns an int status (VT_OK == 0); nothing throws or
+ *    aborts across the ABI.  vt_status_string() names a status.
+ *  - Host arrays handed to vt_world_create_* are copied; the library owns all
+ *    world device memory until vt_world_destroy().
+ *  - Output buffers in vt_frame are caller-owned DEVICE pointers.
+ *  - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *    Work is enqueued asynchronously and in order on that stream, like the
+ *    reference's in-order GL queue plus glMemoryBarrier (shader.zig:116).
+ *  - One vt_ctx per host thread; a vt_ctx is bound to one HIP device.
+ *
+ * Reference semantics ("REF"): the traversal restates traceMap
+ * (assets/shaders/map.glsl:83-168) operation by operation in IEEE-754 binary32
+ * with round-to-nearest-even, no fused multiply-add, correctly rounded
+ * division and square root, so hits are bit-exact against the CPU
+ * restatement.
+ */
+#ifndef VT_H
+#define VT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  VT_OK = 0,
+  VT_ERR_INVALID_ARGUMENT = 1, /* a size, flag or enum value is out of range   */
+  VT_ERR_NULL_POINTER = 2,     /* a required pointer is NULL                   */
+  VT_ERR_OUT_OF_MEMORY = 3,    /* device or host allocation failed             */
+  VT_ERR_DEVICE = 4,           /* a HIP runtime call failed                    */
+  VT_ERR_NO_DEVICE = 5,        /* no HIP device, or the ordinal is invalid     */
+  VT_ERR_UNSUPPORTED = 6,      /* valid request this world/mode cannot serve   */
+  VT_ERR_MISALIGNED = 7,       /* a device pointer is not 16-byte aligned      */
+  VT_ERR_WRONG_CONTEXT = 8     /* a world used with a context it was not made on */
+};
+
+typedef struct vt_ctx vt_ctx;
+typedef struct vt_world vt_world;
+
+/* ---- camera ------------------------------------------------------------ */
+/* Byte-identical to Camera.UniformData (src/engine/graphics/camera.zig:12-16)
+ * and to the std140 block u_Camera (assets/shaders/camera.glsl:2-6):
+ * vec4 C_position @0, mat4 C_view @16 (column-major), float fov @80.     */
+typedef struct vt_camera {
+  float position[4]; /* C_position, xyz used                          */
+  float view[16];    /* C_view, column j at view[4j..4j+3]            */
+  float fov;         /* horizontal field of view in radians           */
+  float _pad[3];
+} vt_camera;
+
+/* ---- render parameters ------------------------------------------------- */
+/* flags */
+#define VT_FLAG_GBUF_UNORM8 (1u << 0) /* G-buffer colour/normal/illumination  */
+                                      /* stored through rgba8 clamp+round,    */
+                                      /* as gbuffer.zig:10-13 declares them   */
+#define VT_FLAG_LAMBERT (1u << 1)     /* shaded = albedo*(ambient+max(n.l,0)*vis) */
+                                      /* instead of the blit composite        */
+
+/* traversal storage variants for dense worlds (identical hits by contract) */
+#define VT_TRAVERSAL_FLAT 0u    /* one uint8 gather per executed step        */
+#define VT_TRAVERSAL_BITMASK 1u /* 2-level uint64 occupancy (4^3 / 16^3)     */
+
+typedef struct vt_render_params {
+  uint32_t width, height;    /* image size in pixels                        */
+  int32_t max_steps_primary; /* traceMap cap, primary.comp.glsl:43 uses 192 */
+  int32_t max_steps_shadow;  /* traceMap cap, secondary.comp.glsl:41 uses 48 */
+  float sun[3];              /* SUN_DIR (camera.glsl:8): shadow-ray direction, */
+                             /* used unnormalised; also the sky's sun         */
+  uint32_t flags;            /* VT_FLAG_*                                     */
+  uint32_t traversal;        /* VT_TRAVERSAL_*                                */
+  float ambient;             /* VT_FLAG_LAMBERT only                          */
+  const float* albedo;       /* VT_FLAG_LAMBERT only: device float[dim^3*3],  */
+                             /* RGB per voxel, index x + dim*(y + dim*z)      */
+} vt_render_params;
+
+/* ---- outputs ------------------------------------------------------------ */
+/* Per-pixel hit record (12 bytes).  voxel = linear block index
+ * x + dim*(y + dim*z) of the hit block (0xFFFFFFFF on a miss); t = distance
+ * from C_position to hit_pos/8; info packs:
+ *   bits 0-2  face id 1..6 (map.glsl:119-125; 0 = miss)
+ *   bit  3    shadow ray occluded (terrain or entity)
+ *   bits 4-12 sub-voxel index x+8y+64z inside the block (brickmap worlds)
+ *   bits 16-23 material id (dense worlds)                                  */
+typedef struct vt_hit_record {
+  uint32_t voxel;
+  float t;
+  uint32_t info;
+} vt_hit_record;
+
+#define VT_INFO_FACE(info) ((info)&7u)
+#define VT_INFO_SHADOWED(info) (((info) >> 3) & 1u)
+#define VT_INFO_SUBVOXEL(info) (((info) >> 4) & 511u)
+#define VT_INFO_MATERIAL(info) (((info) >> 16) & 255u)
+
+/* Statistics accumulated when vt_frame.stats != NULL (device uint64[8]). */
+enum {
+  VT_STAT_PRIMARY_STEPS = 0, /* executed traceMap iterations, primary rays  */
+  VT_STAT_SHADOW_STEPS = 1,  /* executed traceMap iterations, shadow rays   */
+  VT_STAT_PRIMARY_HITS = 2,  /* = number of shadow rays traced              */
+  VT_STAT_OCCLUDED = 3,      /* shadow rays that found an occluder          */
+  VT_STAT_L1_FETCHES = 4,    /* bitmask traversal: 16^3 region words loaded */
+  VT_STAT_L0_FETCHES = 5,    /* bitmask traversal: 4^3 brick words loaded   */
+  VT_STAT_LEAF_READS = 6,    /* voxel bytes / texels read                   */
+  VT_STAT_ENTITY_HITS = 7,   /* shadow rays occluded by traceEntities       */
+  VT_STAT_COUNT = 8
+};
+
+/* Caller-owned device buffers; any may be NULL.  Images are row-major,
+ * row y = pixel y (the reference's gl_GlobalInvocationID.y), 16-byte
+ * aligned. */
+typedef struct vt_frame {
+  float* shaded;         /* W*H*4: RGBA32F composite (or Lambert) result      */
+  vt_hit_record* hits;   /* W*H records                                       */
+  float* color;          /* W*H*4: G-buffer image 0 (frameColor)              */
+  float* normal;         /* W*H*4: G-buffer image 1 (frameNormal)             */
+  float* position;       /* W*H*4: G-buffer image 2 (framePosition)           */
+  float* illumination;   /* W*H*4: G-buffer image 3 (frameIllumination)       */
+  uint32_t* steps;       /* W*H: primary steps | shadow steps << 16           */
+  unsigned long long* stats; /* VT_STAT_COUNT counters, accumulated (+=)      */
+} vt_frame;
+
+/* ---- context ------------------------------------------------------------ */
+/* Replaces the GL context made current in gfx.init (graphics.zig:60-75). */
+int vt_create(vt_ctx** out, int device);
+int vt_destroy(vt_ctx* ctx);
+/* Blocks until all work the library enqueued on `stream` finished. */
+int vt_sync(vt_ctx* ctx, void* stream);
+const char* vt_status_string(int status);
+/* Detail of the last failure on the calling thread (validation message or
+ * HIP error string); "" when none. */
+const char* vt_last_error(void);
+/* Number of compute units of the context's device (for launch sizing). */
+int vt_device_info(const vt_ctx* ctx, int* compute_units, char* name, size_t name_len);
+
+/* ---- worlds -------------------------------------------------------------- */
+/* Dense uint8 world: grid[x + dim*(y + dim*z)], 0 = empty, k>0 = material k.
+ * One grid voxel plays one reference block whose atlas slot is a solid 8^3
+ * cube of colour palette[k] (RGBA8, R in the low byte), so traceMap never
+ * descends to the fine level.  dim must be a multiple of 16, 16..1024, and
+ * palette[1..255] nonzero (a zero texel reads as empty).
+ * Replaces VoxelBrickmap(512,8) + VoxelModelAtlas (src/engine/voxel.zig:25-131)
+ * for the config workloads; also builds the 2-level occupancy bitmasks. */
+int vt_world_create_dense_u8(vt_ctx* ctx, const uint8_t* grid, uint32_t dim,
+                             const uint32_t palette_rgba8[256], vt_world** out);
+
+/* Reference layout (drop-in): chunk table chunks[(dim/8)^3] (0 = empty chunk,
+ * else pool block index + 1), block pool pool[pool_words] (512 u32 per block,
+ * in-block index x + 8y + 64z), atlas_rgba8[256^3] RGBA8 texels, index
+ * x + 256*(y + 256*z).  Exactly the buffers VoxelBrickmap.bind and
+ * VoxelModelAtlas.bind expose (voxel.zig:77-80,129-131); dim = MAP_DIMENSION
+ * (map.glsl:2), a multiple of 8.                                            */
+int vt_world_create_brickmap(vt_ctx* ctx, uint32_t dim, const uint32_t* chunks,
+                             const uint32_t* pool, size_t pool_words,
+                             const uint32_t* atlas_rgba8, vt_world** out);
+int vt_world_destroy(vt_world* world);
+/* dim and device bytes held by the world. */
+int vt_world_info(const vt_world* world, uint32_t* dim, size_t* device_bytes);
+
+/* ---- render ------------------------------------------------------------ */
+/* Primary pass (primary.comp.glsl:23-69) + shadow pass
+ * (secondary.comp.glsl:18-50) + blit composite formula
+ * (blit.fragment.glsl:30-31), fused into one launch.                      */
+int vt_render(vt_ctx* ctx, const vt_world* world, const vt_camera* camera,
+              const vt_render_params* params, const vt_frame* frame, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VT_H */

@@ -1,0 +1,139 @@
+"""ctypes binding of oracle/liboracle.so -- the CPU restatement used as the
+checker and as the CPU baseline.  Test infrastructure only."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+
+
+class OrCamera(C.Structure):
+    _fields_ = [("position", C.c_float * 4), ("view", C.c_float * 16), ("fov", C.c_float),
+                ("_pad", C.c_float * 3)]
+
+
+class OrWorld(C.Structure):
+    _fields_ = [("kind", C.c_int), ("dim", C.c_uint32), ("grid", C.c_void_p),
+                ("palette", C.c_void_p), ("chunks", C.c_void_p), ("pool", C.c_void_p),
+                ("pool_words", C.c_uint64), ("atlas", C.c_void_p), ("count_bitmask", C.c_int)]
+
+
+class OrHit(C.Structure):
+    _fields_ = [("data", C.c_uint32), ("hit_pos", C.c_float * 3), ("normal", C.c_float * 3),
+                ("face", C.c_int), ("block", C.c_int32 * 3), ("sub", C.c_uint32),
+                ("steps", C.c_int), ("tie_margin", C.c_float), ("lookups", C.c_uint32),
+                ("l1_fetches", C.c_uint32), ("l0_fetches", C.c_uint32)]
+
+
+class OrFrame(C.Structure):
+    _fields_ = [("shaded", C.c_void_p), ("hits", C.c_void_p), ("color", C.c_void_p),
+                ("normal", C.c_void_p), ("position", C.c_void_p), ("illumination", C.c_void_p),
+                ("steps", C.c_void_p), ("tie_margin", C.c_void_p), ("stats", C.c_void_p)]
+
+
+class OrParams(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32),
+                ("max_steps_primary", C.c_int32), ("max_steps_shadow", C.c_int32),
+                ("sun", C.c_float * 3), ("flags", C.c_uint32), ("traversal", C.c_uint32),
+                ("ambient", C.c_float), ("albedo", C.c_void_p)]
+
+
+_L = None
+
+
+def lib():
+    global _L
+    if _L is None:
+        path = os.path.join(ORACLE_DIR, "liboracle.so")
+        if not os.path.exists(path):
+            raise OSError(f"{path} missing: run make -C oracle")
+        L = C.CDLL(path)
+        f3 = C.POINTER(C.c_float)
+        L.or_tan_half_fov.restype = C.c_float
+        L.or_tan_half_fov.argtypes = [C.c_float]
+        L.or_raygen.argtypes = [C.POINTER(OrCamera), C.c_float, C.c_uint32, C.c_uint32,
+                                C.c_uint32, C.c_uint32, f3, f3]
+        L.or_intersect_aabb.argtypes = [f3, f3, f3, f3, f3, f3]
+        L.or_entry_origin.argtypes = [f3, f3, C.c_uint32, f3]
+        L.or_trace_map.restype = C.c_int
+        L.or_trace_map.argtypes = [C.POINTER(OrWorld), f3, f3, C.c_int, C.POINTER(OrHit),
+                                   C.c_void_p, C.c_int]
+        L.or_trace_entities.restype = C.c_int
+        L.or_trace_entities.argtypes = [f3, f3, C.c_float]
+        L.or_sky.argtypes = [f3, f3, f3]
+        L.or_render.argtypes = [C.POINTER(OrWorld), C.POINTER(OrCamera), C.POINTER(OrParams),
+                                C.POINTER(OrFrame), C.c_uint32, C.c_uint32, C.c_int]
+        L.or_shade_lambert.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, f3, C.c_float,
+                                       C.c_void_p]
+        L.or_lambert_grad64.restype = C.c_double
+        L.or_lambert_grad64.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, f3, C.c_float,
+                                        C.c_void_p, C.c_void_p, C.c_void_p]
+        _L = L
+    return _L
+
+
+def fvec(v):
+    return (C.c_float * 3)(*[float(np.float32(x)) for x in v])
+
+
+class DenseWorld:
+    """Keeps the numpy buffers alive for the lifetime of the OrWorld."""
+
+    def __init__(self, grid: np.ndarray, palette: np.ndarray):
+        self.grid = np.ascontiguousarray(grid, dtype=np.uint8)
+        self.palette = np.ascontiguousarray(palette, dtype=np.uint32)
+        self.s = OrWorld(0, self.grid.shape[0], self.grid.ctypes.data, self.palette.ctypes.data,
+                         None, None, 0, None, 0)
+
+
+class BrickmapWorld:
+    def __init__(self, dim, chunks, pool, atlas):
+        self.chunks = np.ascontiguousarray(chunks, dtype=np.uint32)
+        self.pool = np.ascontiguousarray(pool, dtype=np.uint32)
+        self.atlas = np.ascontiguousarray(atlas, dtype=np.uint32)
+        self.s = OrWorld(1, dim, None, None, self.chunks.ctypes.data,
+                         self.pool.ctypes.data if self.pool.size else None, self.pool.size,
+                         self.atlas.ctypes.data, 0)
+
+
+def camera(cam) -> OrCamera:
+    c = OrCamera()
+    C.memmove(C.addressof(c), C.addressof(cam), 96)
+    return c
+
+
+def trace(world, o, d, max_steps, path_cap=0):
+    h = OrHit()
+    path = np.zeros((max(path_cap, 1), 3), dtype=np.int32)
+    L = lib()
+    L.or_trace_map(C.byref(world.s), fvec(o), fvec(d), int(max_steps), C.byref(h),
+                   path.ctypes.data if path_cap else None, int(path_cap))
+    return h, path[: min(h.steps, path_cap)] if path_cap else None
+
+
+def render(world, cam, width, height, max_primary, max_shadow, sun, flags=0, traversal=0,
+           ambient=0.0, albedo=None, rows=None, threads=8, gbuffer=True):
+    """Full (or row-range) oracle render; returns a dict of numpy arrays."""
+    p = OrParams(width, height, max_primary, max_shadow, fvec(sun), flags, traversal,
+                 float(ambient), None if albedo is None else albedo.ctypes.data)
+    out = {
+        "shaded": np.zeros((height, width, 4), np.float32),
+        "hits": np.zeros((height, width, 3), np.uint32),
+        "steps": np.zeros((height, width), np.uint32),
+        "tie_margin": np.zeros((height, width), np.float32),
+        "stats": np.zeros(8, np.uint64),
+    }
+    if gbuffer:
+        for k in ("color", "normal", "position", "illumination"):
+            out[k] = np.zeros((height, width, 4), np.float32)
+    ptr = lambda k: out[k].ctypes.data if k in out else None
+    f = OrFrame(ptr("shaded"), ptr("hits"), ptr("color"), ptr("normal"), ptr("position"),
+                ptr("illumination"), ptr("steps"), ptr("tie_margin"), ptr("stats"))
+    y0, y1 = (0, height) if rows is None else rows
+    oc = camera(cam)
+    lib().or_render(C.byref(world.s), C.byref(oc), C.byref(p), C.byref(f), y0, y1, threads)
+    return out

@@ -1,0 +1,64 @@
+"""Build recipe for the in-tree native libraries.
+
+libvt.so         HIP kernels + C-ABI (include/vt.h), gfx950 code objects.
+libvt_scenes.so  host C generators of the synthetic benchmark worlds.
+
+Both land in ``unnamed-voxel-tracer_amd/lib/``.  Run ``python build.py`` or
+call :func:`build`.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+LIB = os.path.join(PKG, "lib")
+CSRC = os.path.join(PKG, "csrc")
+
+HIP_SOURCES = ["vt_api.hip", "vt_render.hip", "vt_world.hip"]
+
+# -ffp-contract=off: every binary32 multiply and add rounds separately (the
+#   traversal must reproduce the CPU restatement bit for bit).
+# correctly rounded divide/sqrt: 1/d, normalize() and length() bit-exact.
+# -packed-fp32-ops: no v_pk_*_f32 in the device code.
+HIP_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    "-ffp-contract=off",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print("+ " + " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(
+        line for line in r.stdout.splitlines()
+        if "packed-fp32-ops' is not a recognized feature" not in line
+    )
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed ({r.returncode}): {' '.join(cmd)}\n{out}")
+    if verbose and out.strip():
+        print(out, flush=True)
+
+
+def build(verbose: bool = True) -> None:
+    os.makedirs(LIB, exist_ok=True)
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    _run([hipcc, *HIP_FLAGS, "-o", os.path.join(LIB, "libvt.so"), *srcs], verbose)
+    _run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-ffp-contract=off", "-Wall",
+          "-o", os.path.join(LIB, "libvt_scenes.so"),
+          os.path.join(PKG, "scenes", "vt_scenes.c"), "-lm"], verbose)
+
+
+if __name__ == "__main__":
+    build(verbose="-q" not in sys.argv)

@@ -1,0 +1,339 @@
+// vt_api.hip -- the extern "C" entry points declared in include/vt.h.
+//
+// Validation happens here, on the host, before anything is enqueued: every
+// size, pointer and alignment a kernel relies on is checked so that no launch
+// can index outside the buffers it is given.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "vt_internal.hpp"
+
+namespace {
+
+thread_local char g_detail[512] = "";
+
+int fail(int status, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int status, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_detail, sizeof g_detail, fmt, ap);
+  va_end(ap);
+  return status;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(VT_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define VT_HIP(call)                                   \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return hip_fail(e_, #call);  \
+  } while (0)
+
+bool aligned(const void* p, size_t a) { return ((uintptr_t)p & (a - 1)) == 0; }
+
+void dev_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+void free_world(vt_world* w) {
+  if (!w) return;
+  dev_free(w->vox);
+  dev_free(w->l0);
+  dev_free(w->l1);
+  dev_free(w->palette);
+  dev_free(w->chunks);
+  dev_free(w->pool);
+  dev_free(w->atlas);
+  delete w;
+}
+
+vt_world* new_world(vt_ctx* ctx, WorldKind kind, uint32_t dim) {
+  vt_world* w = new (std::nothrow) vt_world;
+  if (!w) return nullptr;
+  std::memset(w, 0, sizeof *w);
+  w->ctx = ctx;
+  w->kind = kind;
+  w->dim = dim;
+  return w;
+}
+
+int dev_alloc(vt_world* w, void** p, size_t bytes, const char* what) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(VT_ERR_OUT_OF_MEMORY, "hipMalloc(%zu) for %s: %s", bytes, what,
+                hipGetErrorString(e));
+  }
+  w->bytes += bytes;
+  return VT_OK;
+}
+
+}  // namespace
+
+namespace vt {
+
+DenseWorld dense_view(const vt_world* w) {
+  DenseWorld d;
+  d.vox = w->vox;
+  d.l0 = w->l0;
+  d.l1 = w->l1;
+  d.palette = w->palette;
+  d.dim = w->dim;
+  d.nb = w->dim >> 2;
+  d.nr = w->dim >> 4;
+  return d;
+}
+
+BrickmapWorld brickmap_view(const vt_world* w) {
+  BrickmapWorld b;
+  b.chunks = w->chunks;
+  b.pool = w->pool;
+  b.atlas = w->atlas;
+  b.dim = w->dim;
+  return b;
+}
+
+}  // namespace vt
+
+extern "C" {
+
+const char* vt_status_string(int status) {
+  switch (status) {
+    case VT_OK: return "ok";
+    case VT_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case VT_ERR_NULL_POINTER: return "null pointer";
+    case VT_ERR_OUT_OF_MEMORY: return "out of memory";
+    case VT_ERR_DEVICE: return "HIP runtime error";
+    case VT_ERR_NO_DEVICE: return "no HIP device";
+    case VT_ERR_UNSUPPORTED: return "unsupported for this world or mode";
+    case VT_ERR_MISALIGNED: return "misaligned device pointer";
+    case VT_ERR_WRONG_CONTEXT: return "world belongs to another context";
+    default: return "unknown status";
+  }
+}
+
+const char* vt_last_error(void) { return g_detail; }
+
+int vt_create(vt_ctx** out, int device) {
+  if (!out) return fail(VT_ERR_NULL_POINTER, "vt_create: out is NULL");
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    return fail(VT_ERR_NO_DEVICE, "vt_create: no HIP device (%s)", hipGetErrorString(e));
+  }
+  if (device < 0 || device >= n)
+    return fail(VT_ERR_NO_DEVICE, "vt_create: device %d of %d", device, n);
+  VT_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  VT_HIP(hipGetDeviceProperties(&prop, device));
+  vt_ctx* c = new (std::nothrow) vt_ctx;
+  if (!c) return fail(VT_ERR_OUT_OF_MEMORY, "vt_create: host allocation");
+  c->device = device;
+  c->compute_units = prop.multiProcessorCount;
+  std::snprintf(c->name, sizeof c->name, "%s", prop.gcnArchName);
+  *out = c;
+  return VT_OK;
+}
+
+int vt_destroy(vt_ctx* ctx) {
+  if (!ctx) return fail(VT_ERR_NULL_POINTER, "vt_destroy: ctx is NULL");
+  delete ctx;
+  return VT_OK;
+}
+
+int vt_sync(vt_ctx* ctx, void* stream) {
+  if (!ctx) return fail(VT_ERR_NULL_POINTER, "vt_sync: ctx is NULL");
+  VT_HIP(hipSetDevice(ctx->device));
+  VT_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return VT_OK;
+}
+
+int vt_device_info(const vt_ctx* ctx, int* compute_units, char* name, size_t name_len) {
+  if (!ctx) return fail(VT_ERR_NULL_POINTER, "vt_device_info: ctx is NULL");
+  if (compute_units) *compute_units = ctx->compute_units;
+  if (name && name_len) std::snprintf(name, name_len, "%s", ctx->name);
+  return VT_OK;
+}
+
+int vt_world_create_dense_u8(vt_ctx* ctx, const uint8_t* grid, uint32_t dim,
+                             const uint32_t palette_rgba8[256], vt_world** out) {
+  if (!out) return fail(VT_ERR_NULL_POINTER, "vt_world_create_dense_u8: out is NULL");
+  *out = nullptr;
+  if (!ctx || !grid || !palette_rgba8)
+    return fail(VT_ERR_NULL_POINTER, "vt_world_create_dense_u8: ctx, grid and palette required");
+  if (dim < 16 || dim > 1024 || dim % 16 != 0)
+    return fail(VT_ERR_INVALID_ARGUMENT,
+                "vt_world_create_dense_u8: dim %u must be a multiple of 16 in [16, 1024]", dim);
+  for (int k = 1; k < 256; k++)
+    if (palette_rgba8[k] == 0u)
+      return fail(VT_ERR_INVALID_ARGUMENT,
+                  "vt_world_create_dense_u8: palette[%d] is 0 (a material colour must be "
+                  "nonzero: a zero texel reads as empty)",
+                  k);
+  VT_HIP(hipSetDevice(ctx->device));
+  vt_world* w = new_world(ctx, WorldKind::Dense, dim);
+  if (!w) return fail(VT_ERR_OUT_OF_MEMORY, "vt_world_create_dense_u8: host allocation");
+  const size_t n = (size_t)dim * dim * dim;
+  const size_t nb = (size_t)(dim / 4) * (dim / 4) * (dim / 4);
+  const size_t nr = (size_t)(dim / 16) * (dim / 16) * (dim / 16);
+  int st;
+  uint8_t* lin = nullptr;
+  if ((st = dev_alloc(w, (void**)&w->vox, n, "voxels")) != VT_OK ||
+      (st = dev_alloc(w, (void**)&w->l0, nb * 8, "L0 masks")) != VT_OK ||
+      (st = dev_alloc(w, (void**)&w->l1, nr * 8, "L1 masks")) != VT_OK ||
+      (st = dev_alloc(w, (void**)&w->palette, 256 * 4, "palette")) != VT_OK) {
+    free_world(w);
+    return st;
+  }
+  hipError_t e = hipMalloc((void**)&lin, n);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    free_world(w);
+    return fail(VT_ERR_OUT_OF_MEMORY, "vt_world_create_dense_u8: staging %zu bytes", n);
+  }
+  e = hipMemcpy(lin, grid, n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(w->palette, palette_rgba8, 256 * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = vt::launch_build_dense(lin, dim, w->vox, w->l0, w->l1, nullptr);
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  dev_free(lin);
+  if (e != hipSuccess) {
+    free_world(w);
+    return hip_fail(e, "vt_world_create_dense_u8: upload/build");
+  }
+  *out = w;
+  return VT_OK;
+}
+
+int vt_world_create_brickmap(vt_ctx* ctx, uint32_t dim, const uint32_t* chunks,
+                             const uint32_t* pool, size_t pool_words,
+                             const uint32_t* atlas_rgba8, vt_world** out) {
+  if (!out) return fail(VT_ERR_NULL_POINTER, "vt_world_create_brickmap: out is NULL");
+  *out = nullptr;
+  if (!ctx || !chunks || !atlas_rgba8 || (!pool && pool_words))
+    return fail(VT_ERR_NULL_POINTER, "vt_world_create_brickmap: ctx, chunks, atlas required");
+  if (dim < 8 || dim > 1024 || dim % 8 != 0)
+    return fail(VT_ERR_INVALID_ARGUMENT,
+                "vt_world_create_brickmap: dim %u must be a multiple of 8 in [8, 1024]", dim);
+  if (pool_words % 512 != 0)
+    return fail(VT_ERR_INVALID_ARGUMENT,
+                "vt_world_create_brickmap: pool_words %zu is not a whole number of 8^3 blocks",
+                pool_words);
+  const size_t M = dim / 8, nchunks = M * M * M, nblocks = pool_words / 512;
+  for (size_t i = 0; i < nchunks; i++)
+    if (chunks[i] > nblocks)
+      return fail(VT_ERR_INVALID_ARGUMENT,
+                  "vt_world_create_brickmap: chunk %zu names pool block %u of %zu", i,
+                  chunks[i] - 1u, nblocks);
+  VT_HIP(hipSetDevice(ctx->device));
+  vt_world* w = new_world(ctx, WorldKind::Brickmap, dim);
+  if (!w) return fail(VT_ERR_OUT_OF_MEMORY, "vt_world_create_brickmap: host allocation");
+  const size_t atlas_words = 256u * 256u * 256u;
+  int st;
+  if ((st = dev_alloc(w, (void**)&w->chunks, nchunks * 4, "chunk table")) != VT_OK ||
+      (st = dev_alloc(w, (void**)&w->pool, (pool_words ? pool_words : 512) * 4, "block pool")) !=
+          VT_OK ||
+      (st = dev_alloc(w, (void**)&w->atlas, atlas_words * 4, "model atlas")) != VT_OK) {
+    free_world(w);
+    return st;
+  }
+  w->pool_words = pool_words;
+  hipError_t e = hipMemcpy(w->chunks, chunks, nchunks * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess && pool_words)
+    e = hipMemcpy(w->pool, pool, pool_words * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(w->atlas, atlas_rgba8, atlas_words * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    free_world(w);
+    return hip_fail(e, "vt_world_create_brickmap: upload");
+  }
+  *out = w;
+  return VT_OK;
+}
+
+int vt_world_destroy(vt_world* world) {
+  if (!world) return fail(VT_ERR_NULL_POINTER, "vt_world_destroy: world is NULL");
+  (void)hipSetDevice(world->ctx->device);
+  free_world(world);
+  return VT_OK;
+}
+
+int vt_world_info(const vt_world* world, uint32_t* dim, size_t* device_bytes) {
+  if (!world) return fail(VT_ERR_NULL_POINTER, "vt_world_info: world is NULL");
+  if (dim) *dim = world->dim;
+  if (device_bytes) *device_bytes = world->bytes;
+  return VT_OK;
+}
+
+int vt_render(vt_ctx* ctx, const vt_world* world, const vt_camera* camera,
+              const vt_render_params* params, const vt_frame* frame, void* stream) {
+  if (!ctx || !world || !camera || !params || !frame)
+    return fail(VT_ERR_NULL_POINTER, "vt_render: ctx, world, camera, params, frame required");
+  if (world->ctx != ctx) return fail(VT_ERR_WRONG_CONTEXT, "vt_render: world from another ctx");
+  const vt_render_params& p = *params;
+  if (p.width == 0 || p.height == 0 || p.width > 16384 || p.height > 16384)
+    return fail(VT_ERR_INVALID_ARGUMENT, "vt_render: image %ux%u outside [1,16384]^2", p.width,
+                p.height);
+  if (p.max_steps_primary < 0 || p.max_steps_primary > 65535 || p.max_steps_shadow < 0 ||
+      p.max_steps_shadow > 65535)
+    return fail(VT_ERR_INVALID_ARGUMENT, "vt_render: step caps must lie in [0, 65535]");
+  if (p.flags & ~(VT_FLAG_GBUF_UNORM8 | VT_FLAG_LAMBERT))
+    return fail(VT_ERR_INVALID_ARGUMENT, "vt_render: unknown flags 0x%x", p.flags);
+  if (p.traversal > VT_TRAVERSAL_BITMASK)
+    return fail(VT_ERR_INVALID_ARGUMENT, "vt_render: unknown traversal %u", p.traversal);
+  if (world->kind == WorldKind::Brickmap && p.traversal != VT_TRAVERSAL_FLAT)
+    return fail(VT_ERR_UNSUPPORTED, "vt_render: brickmap worlds trace with VT_TRAVERSAL_FLAT");
+  if ((p.flags & VT_FLAG_LAMBERT) && (!p.albedo || world->kind != WorldKind::Dense))
+    return fail(VT_ERR_UNSUPPORTED, "vt_render: VT_FLAG_LAMBERT needs a dense world and albedo");
+  if (p.albedo && !aligned(p.albedo, 4)) return fail(VT_ERR_MISALIGNED, "vt_render: albedo");
+  const void* f16[] = {frame->shaded, frame->color, frame->normal, frame->position,
+                       frame->illumination};
+  for (const void* q : f16)
+    if (q && !aligned(q, 16)) return fail(VT_ERR_MISALIGNED, "vt_render: image not 16B aligned");
+  if ((frame->hits && !aligned(frame->hits, 4)) || (frame->steps && !aligned(frame->steps, 4)) ||
+      (frame->stats && !aligned(frame->stats, 8)))
+    return fail(VT_ERR_MISALIGNED, "vt_render: hits/steps/stats misaligned");
+  for (int i = 0; i < 3; i++)
+    if (!std::isfinite(p.sun[i])) return fail(VT_ERR_INVALID_ARGUMENT, "vt_render: sun not finite");
+
+  VT_HIP(hipSetDevice(ctx->device));
+  vt::ViewConsts v;
+  std::memcpy(v.pos, camera->position, sizeof v.pos);
+  std::memcpy(v.view, camera->view, sizeof v.view);
+  v.tan_half = (float)std::tan((double)(camera->fov / 2.0f));
+  v.W = p.width;
+  v.H = p.height;
+  vt::ShadeConsts sc;
+  std::memcpy(sc.sun, p.sun, sizeof sc.sun);
+  sc.max_primary = p.max_steps_primary;
+  sc.max_shadow = p.max_steps_shadow;
+  sc.flags = p.flags;
+  sc.ambient = p.ambient;
+  sc.albedo = p.albedo;
+  vt::FrameOut o;
+  o.shaded = reinterpret_cast<float4*>(frame->shaded);
+  o.hits = reinterpret_cast<uint32_t*>(frame->hits);
+  o.color = reinterpret_cast<float4*>(frame->color);
+  o.normal = reinterpret_cast<float4*>(frame->normal);
+  o.position = reinterpret_cast<float4*>(frame->position);
+  o.illum = reinterpret_cast<float4*>(frame->illumination);
+  o.steps = frame->steps;
+  o.stats = frame->stats;
+  vt::TileShape tile{3};
+  hipError_t e;
+  if (world->kind == WorldKind::Dense)
+    e = vt::launch_render_dense(vt::dense_view(world), p.traversal, v, sc, o, tile,
+                                (hipStream_t)stream);
+  else
+    e = vt::launch_render_brickmap(vt::brickmap_view(world), v, sc, o, tile, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "vt_render: launch");
+  return VT_OK;
+}
+
+}  // extern "C"

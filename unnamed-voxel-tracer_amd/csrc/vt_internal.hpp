@@ -1,0 +1,80 @@
+// vt_internal.hpp -- host-side objects behind the C-ABI handles and the
+// launchers each kernel translation unit exports.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/vt.h"
+#include "vt_trace.hpp"
+
+struct vt_ctx {
+  int device;
+  int compute_units;
+  char name[256];
+};
+
+enum class WorldKind { Dense, Brickmap };
+
+struct vt_world {
+  vt_ctx* ctx;
+  WorldKind kind;
+  uint32_t dim;
+  size_t bytes;
+  // dense
+  uint8_t* vox;
+  uint64_t* l0;
+  uint64_t* l1;
+  uint32_t* palette;
+  // brickmap
+  uint32_t* chunks;
+  uint32_t* pool;
+  uint32_t* atlas;
+  size_t pool_words;
+};
+
+namespace vt {
+
+// Per-launch shading constants.
+struct ShadeConsts {
+  float sun[3];
+  int max_primary, max_shadow;
+  uint32_t flags;
+  float ambient;
+  const float* albedo;
+};
+
+// Output pointers (any may be null).
+struct FrameOut {
+  float4* shaded;
+  uint32_t* hits;  // 3 words per pixel
+  float4* color;
+  float4* normal;
+  float4* position;
+  float4* illum;
+  uint32_t* steps;
+  unsigned long long* stats;
+};
+
+// Wave tile: 64 lanes cover tile_w x (64 / tile_w) pixels; a 256-thread
+// block covers 2 x 2 wave tiles.
+struct TileShape {
+  uint32_t log2_w;  // 3 (8x8), 4 (16x4) or 5 (32x2)
+};
+
+DenseWorld dense_view(const vt_world* w);
+BrickmapWorld brickmap_view(const vt_world* w);
+
+hipError_t launch_render_dense(const DenseWorld& w, uint32_t traversal, const ViewConsts& v,
+                               const ShadeConsts& sc, const FrameOut& out, TileShape tile,
+                               hipStream_t stream);
+hipError_t launch_render_brickmap(const BrickmapWorld& w, const ViewConsts& v,
+                                  const ShadeConsts& sc, const FrameOut& out, TileShape tile,
+                                  hipStream_t stream);
+
+// World construction kernels.
+hipError_t launch_build_dense(const uint8_t* linear, uint32_t dim, uint8_t* vox, uint64_t* l0,
+                              uint64_t* l1, hipStream_t stream);
+
+}  // namespace vt

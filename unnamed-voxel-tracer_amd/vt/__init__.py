@@ -1,0 +1,6 @@
+"""MI355X voxel ray traversal: Python host side over libvt.so (include/vt.h)."""
+from . import _native, camera, scenes  # noqa: F401
+from ._native import (VT_FLAG_GBUF_UNORM8, VT_FLAG_LAMBERT, VT_TRAVERSAL_BITMASK,  # noqa: F401
+                      VT_TRAVERSAL_FLAT, VtError, lib)
+from .camera import look_at  # noqa: F401
+from .render import Context, Frame, World, params, render  # noqa: F401

@@ -1,0 +1,163 @@
+"""Host-side handles over the C-ABI: contexts, worlds, frames, render calls.
+
+Mirrors the reference's render boundary: ``Game.render`` (src/game.zig:232-256)
+binds the camera UBO, the voxel brickmap SSBOs, the model atlas and the
+G-buffer images, then dispatches the primary and shadow passes.  Here a
+``World`` owns the device copy of the voxel data, a ``Frame`` owns the output
+images (torch CUDA tensors are only the device-memory plumbing) and
+``render`` enqueues the fused HIP launch on the current torch stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+from .scenes import SUN
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("vt needs a HIP device (torch.cuda.is_available() is False)")
+    return torch
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.device = device
+        h = C.c_void_p()
+        N.check(N.lib().vt_create(C.byref(h), device), "vt_create")
+        self.h = h
+
+    def info(self):
+        cu = C.c_int()
+        name = C.create_string_buffer(256)
+        N.check(N.lib().vt_device_info(self.h, C.byref(cu), name, 256), "vt_device_info")
+        return {"compute_units": cu.value, "arch": name.value.decode()}
+
+    def close(self):
+        if self.h:
+            N.check(N.lib().vt_destroy(self.h), "vt_destroy")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class World:
+    def __init__(self, ctx: Context, h: C.c_void_p, dim: int, kind: str):
+        self.ctx, self.h, self.dim, self.kind = ctx, h, dim, kind
+
+    @classmethod
+    def dense(cls, ctx: Context, grid: np.ndarray, palette: np.ndarray) -> "World":
+        grid = np.ascontiguousarray(grid, dtype=np.uint8)
+        dim = grid.shape[0]
+        assert grid.shape == (dim, dim, dim)
+        pal = np.ascontiguousarray(palette, dtype=np.uint32)
+        assert pal.shape == (256,)
+        h = C.c_void_p()
+        N.check(N.lib().vt_world_create_dense_u8(ctx.h, grid.ctypes.data, dim, pal.ctypes.data,
+                                                 C.byref(h)), "vt_world_create_dense_u8")
+        return cls(ctx, h, dim, "dense")
+
+    @classmethod
+    def brickmap(cls, ctx: Context, dim: int, chunks: np.ndarray, pool: np.ndarray,
+                 atlas: np.ndarray) -> "World":
+        chunks = np.ascontiguousarray(chunks, dtype=np.uint32)
+        pool = np.ascontiguousarray(pool, dtype=np.uint32)
+        atlas = np.ascontiguousarray(atlas, dtype=np.uint32)
+        h = C.c_void_p()
+        N.check(N.lib().vt_world_create_brickmap(ctx.h, dim, chunks.ctypes.data,
+                                                 pool.ctypes.data if pool.size else None,
+                                                 pool.size, atlas.ctypes.data, C.byref(h)),
+                "vt_world_create_brickmap")
+        return cls(ctx, h, dim, "brickmap")
+
+    def device_bytes(self) -> int:
+        d, b = C.c_uint32(), C.c_size_t()
+        N.check(N.lib().vt_world_info(self.h, C.byref(d), C.byref(b)), "vt_world_info")
+        return b.value
+
+    def close(self):
+        if self.h:
+            N.check(N.lib().vt_world_destroy(self.h), "vt_world_destroy")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class Frame:
+    """Device output images of one view (torch tensors on the context's GPU)."""
+    width: int
+    height: int
+    shaded: object = None
+    hits: object = None
+    color: object = None
+    normal: object = None
+    position: object = None
+    illumination: object = None
+    steps: object = None
+    stats: object = None
+
+    @classmethod
+    def alloc(cls, width, height, device=0, shaded=True, hits=True, gbuffer=False, steps=False,
+              stats=False):
+        torch = _torch()
+        dev = f"cuda:{device}"
+        f = cls(width, height)
+        img = lambda: torch.empty((height, width, 4), dtype=torch.float32, device=dev)
+        if shaded:
+            f.shaded = img()
+        if hits:
+            f.hits = torch.empty((height, width, 3), dtype=torch.int32, device=dev)
+        if gbuffer:
+            f.color, f.normal, f.position, f.illumination = img(), img(), img(), img()
+        if steps:
+            f.steps = torch.empty((height, width), dtype=torch.int32, device=dev)
+        if stats:
+            f.stats = torch.zeros(8, dtype=torch.int64, device=dev)
+        return f
+
+    def struct(self) -> N.Frame:
+        p = lambda t: None if t is None else t.data_ptr()
+        return N.Frame(p(self.shaded), p(self.hits), p(self.color), p(self.normal),
+                       p(self.position), p(self.illumination), p(self.steps), p(self.stats))
+
+    def stats_dict(self):
+        s = self.stats.cpu().numpy()
+        return {k: int(v) for k, v in zip(N.STAT_NAMES, s)}
+
+
+def params(width, height, max_primary=512, max_shadow=512, sun=SUN, flags=0, traversal=0,
+           ambient=0.0, albedo=None) -> N.RenderParams:
+    p = N.RenderParams()
+    p.width, p.height = width, height
+    p.max_steps_primary, p.max_steps_shadow = max_primary, max_shadow
+    p.sun[:] = [float(np.float32(x)) for x in sun]
+    p.flags, p.traversal, p.ambient = flags, traversal, float(ambient)
+    p.albedo = None if albedo is None else albedo.data_ptr()
+    return p
+
+
+def current_stream():
+    return _torch().cuda.current_stream().cuda_stream
+
+
+def render(ctx: Context, world: World, cam: N.Camera, frame: Frame, prm: N.RenderParams,
+           stream=None) -> Frame:
+    fs = frame.struct()
+    s = current_stream() if stream is None else stream
+    N.check(N.lib().vt_render(ctx.h, world.h, C.byref(cam), C.byref(prm), C.byref(fs),
+                              C.c_void_p(s)), "vt_render")
+    return frame
