@@ -26,10 +26,16 @@ OUT_BYTES_PER_PIXEL = 16 + 12  # RGBA32F shaded + 12-byte hit record
 
 
 def algorithmic_bytes(stats, traversal, npix):
+    """Bytes one launch must move: the traversal's gathers plus the outputs.
+
+    flat:    1 byte per executed traceMap step (the voxel gather);
+    bitmask: 8 B per 16^3-region word + 8 B per 4^3-brick word fetched,
+             + 1 B material per primary hit.
+    """
     if traversal == "bitmask":
         b = 8 * stats["l1_fetches"] + 8 * stats["l0_fetches"] + stats["leaf_reads"]
     else:
-        b = stats["leaf_reads"]
+        b = stats["primary_steps"] + stats["shadow_steps"]
     return b + OUT_BYTES_PER_PIXEL * npix
 
 

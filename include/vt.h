@@ -114,7 +114,9 @@ enum {
   VT_STAT_OCCLUDED = 3,      /* shadow rays that found an occluder          */
   VT_STAT_L1_FETCHES = 4,    /* bitmask traversal: 16^3 region words loaded */
   VT_STAT_L0_FETCHES = 5,    /* bitmask traversal: 4^3 brick words loaded   */
-  VT_STAT_LEAF_READS = 6,    /* voxel bytes / texels read                   */
+  VT_STAT_LEAF_READS = 6,    /* bitmask traversal: material bytes read at   */
+                             /* primary hits (flat: 0, its gathers are one  */
+                             /* byte per executed step)                     */
   VT_STAT_ENTITY_HITS = 7,   /* shadow rays occluded by traceEntities       */
   VT_STAT_COUNT = 8
 };
@@ -150,8 +152,10 @@ int vt_device_info(const vt_ctx* ctx, int* compute_units, char* name, size_t nam
 /* Dense uint8 world: grid[x + dim*(y + dim*z)], 0 = empty, k>0 = material k.
  * One grid voxel plays one reference block whose atlas slot is a solid 8^3
  * cube of colour palette[k] (RGBA8, R in the low byte), so traceMap never
- * descends to the fine level.  dim must be a multiple of 16, 16..1024, and
- * palette[1..255] nonzero (a zero texel reads as empty).
+ * descends to the fine level.  dim must be a power of two, 16..1024
+ * (MAP_DIMENSION is 512 in map.glsl:2), and palette[1..255] nonzero (a zero
+ * texel reads as empty).  On the device the voxels are stored in Morton
+ * order with 4^3-brick / 16^3-region occupancy bitmasks.
  * Replaces VoxelBrickmap(512,8) + VoxelModelAtlas (src/engine/voxel.zig:25-131)
  * for the config workloads; also builds the 2-level occupancy bitmasks. */
 int vt_world_create_dense_u8(vt_ctx* ctx, const uint8_t* grid, uint32_t dim,

@@ -156,38 +156,6 @@ static inline uint32_t get_sub(const or_world* w, uint32_t block, int32_t sx, in
 static const float NORMALS[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0},
                                     {0, 1, 0},  {0, 0, -1}, {0, 0, 1}};
 
-/* Memory model of the bitmask traversal (counted for algorithmic bytes):
- * the kernel keeps the last 16^3-region word and the last 4^3-brick word of
- * each ray in registers and fetches a word only when the region / brick of
- * the looked-up block changes (a brick whose region bit is clear is known
- * empty without a fetch). */
-typedef struct mm_cache {
-  int64_t region, brick;
-  int brick_nonempty;
-} mm_cache;
-
-static inline void mm_lookup(const or_world* w, mm_cache* c, int32_t bx, int32_t by, int32_t bz,
-                             or_hit* h) {
-  int64_t rk = ((int64_t)(bx >> 4) << 40) | ((int64_t)(by >> 4) << 20) | (int64_t)(bz >> 4);
-  int64_t bk = ((int64_t)(bx >> 2) << 40) | ((int64_t)(by >> 2) << 20) | (int64_t)(bz >> 2);
-  if (rk != c->region) {
-    c->region = rk;
-    c->brick = -1;
-    h->l1_fetches++;
-  }
-  if (bk != c->brick) {
-    c->brick = bk;
-    /* region word bit for this brick: any voxel of the 4^3 brick solid */
-    int ne = 0;
-    for (int z = 0; z < 4 && !ne; z++)
-      for (int y = 0; y < 4 && !ne; y++)
-        for (int x = 0; x < 4 && !ne; x++)
-          ne = get_block(w, (bx & ~3) + x, (by & ~3) + y, (bz & ~3) + z) != 0;
-    c->brick_nonempty = ne;
-    if (ne) h->l0_fetches++;
-  }
-}
-
 /* traceMap (map.glsl:83-168), REF arithmetic. */
 int or_trace_map(const or_world* w, const float o[3], const float d_in[3], int max_steps,
                  or_hit* hit, int32_t* path, int path_cap) {
@@ -213,7 +181,6 @@ int or_trace_map(const or_world* w, const float o[3], const float d_in[3], int m
     wc[i] = s8 - (float)g[i];
   }
   int s = 0;
-  mm_cache cache = {-1, -1, 0};
 
   memset(hit, 0, sizeof(*hit));
   hit->tie_margin = INFINITY;
@@ -234,7 +201,6 @@ int or_trace_map(const or_world* w, const float o[3], const float d_in[3], int m
     if (w->kind == OR_WORLD_DENSE && b[0] >= 0 && b[1] >= 0 && b[2] >= 0 &&
         b[0] < (int32_t)w->dim && b[1] < (int32_t)w->dim && b[2] < (int32_t)w->dim) {
       hit->lookups++;
-      if (w->count_bitmask) mm_lookup(w, &cache, b[0], b[1], b[2], hit);
     }
     uint32_t block = get_block(w, b[0], b[1], b[2]);
     if (block != 0) {
@@ -377,13 +343,10 @@ static void render_pixel(job* J, uint32_t px, uint32_t py, uint64_t st[8]) {
   int hit = or_trace_map(w, e, d, p->max_steps_primary, &h, NULL, 0);
   const int bitmask = p->traversal == 1u;
   st[0] += (uint64_t)h.steps;
-  if (bitmask) { /* word fetches; the material byte is read once at a hit */
-    st[4] += h.l1_fetches;
-    st[5] += h.l0_fetches;
-    st[6] += (uint64_t)hit;
-  } else { /* flat traversal: one gather per in-bounds lookup */
-    st[6] += h.lookups;
-  }
+  /* bitmask traversal: the material byte is read once at a primary hit (its
+   * word fetches, stats 4 and 5, depend on the kernel's cache and are counted
+   * by the kernel alone); flat traversal: one gather per step, = steps. */
+  if (bitmask) st[6] += (uint64_t)hit;
 
   float color[4], normal[4], position[4];
   if (hit) {
@@ -430,12 +393,6 @@ static void render_pixel(job* J, uint32_t px, uint32_t py, uint64_t st[8]) {
     st[1] += (uint64_t)sh.steps;
     st[2] += 1;
     st[3] += (uint64_t)occluded;
-    if (bitmask) {
-      st[4] += sh.l1_fetches;
-      st[5] += sh.l0_fetches;
-    } else {
-      st[6] += sh.lookups;
-    }
     st[7] += (uint64_t)ent;
     illum[0] = p->sun[0];
     illum[1] = p->sun[1];
@@ -524,9 +481,7 @@ static void* worker(void* arg) {
 void or_render(const or_world* w, const or_camera* cam, const or_params* p, const or_frame* f,
                uint32_t y0, uint32_t y1, int threads) {
   job J;
-  or_world wc = *w;
-  wc.count_bitmask = p->traversal == 1u;
-  J.w = &wc;
+  J.w = w;
   J.cam = cam;
   J.p = p;
   J.f = f;

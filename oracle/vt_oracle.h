@@ -55,7 +55,6 @@ typedef struct or_world {
   const uint32_t* pool;
   uint64_t pool_words;
   const uint32_t* atlas;
-  int count_bitmask; /* 1: count the bitmask traversal's word fetches */
 } or_world;
 
 /* HitInfo (map.glsl:62-70) plus bookkeeping. */
@@ -68,9 +67,7 @@ typedef struct or_hit {
   uint32_t sub;      /* pos % 8 packed x + 8y + 64z                   */
   int steps;         /* executed loop iterations                      */
   float tie_margin;  /* min over DDA steps of (2nd smallest - smallest t) */
-  /* memory model counters (bitmask variant of dense worlds)          */
-  uint32_t lookups;  /* in-bounds block lookups                       */
-  uint32_t l1_fetches, l0_fetches;
+  uint32_t lookups;  /* in-bounds block lookups (dense worlds)        */
 } or_hit;
 
 /* tan(fov/2) as evaluated once per view. */

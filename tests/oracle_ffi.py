@@ -19,14 +19,13 @@ class OrCamera(C.Structure):
 class OrWorld(C.Structure):
     _fields_ = [("kind", C.c_int), ("dim", C.c_uint32), ("grid", C.c_void_p),
                 ("palette", C.c_void_p), ("chunks", C.c_void_p), ("pool", C.c_void_p),
-                ("pool_words", C.c_uint64), ("atlas", C.c_void_p), ("count_bitmask", C.c_int)]
+                ("pool_words", C.c_uint64), ("atlas", C.c_void_p)]
 
 
 class OrHit(C.Structure):
     _fields_ = [("data", C.c_uint32), ("hit_pos", C.c_float * 3), ("normal", C.c_float * 3),
                 ("face", C.c_int), ("block", C.c_int32 * 3), ("sub", C.c_uint32),
-                ("steps", C.c_int), ("tie_margin", C.c_float), ("lookups", C.c_uint32),
-                ("l1_fetches", C.c_uint32), ("l0_fetches", C.c_uint32)]
+                ("steps", C.c_int), ("tie_margin", C.c_float), ("lookups", C.c_uint32)]
 
 
 class OrFrame(C.Structure):
@@ -87,7 +86,7 @@ class DenseWorld:
         self.grid = np.ascontiguousarray(grid, dtype=np.uint8)
         self.palette = np.ascontiguousarray(palette, dtype=np.uint32)
         self.s = OrWorld(0, self.grid.shape[0], self.grid.ctypes.data, self.palette.ctypes.data,
-                         None, None, 0, None, 0)
+                         None, None, 0, None)
 
 
 class BrickmapWorld:
@@ -97,7 +96,7 @@ class BrickmapWorld:
         self.atlas = np.ascontiguousarray(atlas, dtype=np.uint32)
         self.s = OrWorld(1, dim, None, None, self.chunks.ctypes.data,
                          self.pool.ctypes.data if self.pool.size else None, self.pool.size,
-                         self.atlas.ctypes.data, 0)
+                         self.atlas.ctypes.data)
 
 
 def camera(cam) -> OrCamera:

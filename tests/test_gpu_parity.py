@@ -52,7 +52,12 @@ def compare(g, o, rows=None, stats=True, label=""):
     err = np.max(np.abs(g["shaded"][sl] - o["shaded"][sl]))
     assert err <= RGB_TOL, f"{label}: shaded max-abs {err}"
     if stats:
-        assert np.array_equal(g["stats"], o["stats"]), f"{label}: stats {g['stats']} vs {o['stats']}"
+        # steps, primary hits (= shadow rays), occluded, material reads, entity
+        # hits; counters 4-5 (bitmask word fetches) are the kernel's own
+        # cache bookkeeping and have no reference counterpart
+        sel = [0, 1, 2, 3, 6, 7]
+        assert np.array_equal(g["stats"][sel], o["stats"][sel]), \
+            f"{label}: stats {g['stats']} vs {o['stats']}"
 
 
 @pytest.fixture(scope="module")

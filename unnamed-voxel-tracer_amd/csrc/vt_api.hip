@@ -85,8 +85,12 @@ DenseWorld dense_view(const vt_world* w) {
   d.l1 = w->l1;
   d.palette = w->palette;
   d.dim = w->dim;
-  d.nb = w->dim >> 2;
-  d.nr = w->dim >> 4;
+  uint32_t bits = 0;
+  while ((1u << bits) < w->dim) bits++;
+  const uint32_t low = bits * 3 >= 32 ? 0xFFFFFFFFu : ((1u << (bits * 3)) - 1u);
+  d.mx = 0x09249249u & low;
+  d.my = (0x09249249u << 1) & low;
+  d.mz = (0x09249249u << 2) & low;
   return d;
 }
 
@@ -169,9 +173,9 @@ int vt_world_create_dense_u8(vt_ctx* ctx, const uint8_t* grid, uint32_t dim,
   *out = nullptr;
   if (!ctx || !grid || !palette_rgba8)
     return fail(VT_ERR_NULL_POINTER, "vt_world_create_dense_u8: ctx, grid and palette required");
-  if (dim < 16 || dim > 1024 || dim % 16 != 0)
+  if (dim < 16 || dim > 1024 || (dim & (dim - 1)) != 0)
     return fail(VT_ERR_INVALID_ARGUMENT,
-                "vt_world_create_dense_u8: dim %u must be a multiple of 16 in [16, 1024]", dim);
+                "vt_world_create_dense_u8: dim %u must be a power of two in [16, 1024]", dim);
   for (int k = 1; k < 256; k++)
     if (palette_rgba8[k] == 0u)
       return fail(VT_ERR_INVALID_ARGUMENT,
@@ -182,17 +186,15 @@ int vt_world_create_dense_u8(vt_ctx* ctx, const uint8_t* grid, uint32_t dim,
   vt_world* w = new_world(ctx, WorldKind::Dense, dim);
   if (!w) return fail(VT_ERR_OUT_OF_MEMORY, "vt_world_create_dense_u8: host allocation");
   const size_t n = (size_t)dim * dim * dim;
-  const size_t nb = (size_t)(dim / 4) * (dim / 4) * (dim / 4);
-  const size_t nr = (size_t)(dim / 16) * (dim / 16) * (dim / 16);
   int st;
-  uint8_t* lin = nullptr;
   if ((st = dev_alloc(w, (void**)&w->vox, n, "voxels")) != VT_OK ||
-      (st = dev_alloc(w, (void**)&w->l0, nb * 8, "L0 masks")) != VT_OK ||
-      (st = dev_alloc(w, (void**)&w->l1, nr * 8, "L1 masks")) != VT_OK ||
+      (st = dev_alloc(w, (void**)&w->l0, n / 8, "L0 masks")) != VT_OK ||
+      (st = dev_alloc(w, (void**)&w->l1, n / 512, "L1 masks")) != VT_OK ||
       (st = dev_alloc(w, (void**)&w->palette, 256 * 4, "palette")) != VT_OK) {
     free_world(w);
     return st;
   }
+  uint8_t* lin = nullptr;
   hipError_t e = hipMalloc((void**)&lin, n);
   if (e != hipSuccess) {
     (void)hipGetLastError();

@@ -22,7 +22,7 @@ struct vt_world {
   WorldKind kind;
   uint32_t dim;
   size_t bytes;
-  // dense
+  // dense (Morton order)
   uint8_t* vox;
   uint64_t* l0;
   uint64_t* l1;

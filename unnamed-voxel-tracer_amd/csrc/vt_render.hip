@@ -176,19 +176,14 @@ __global__ __launch_bounds__(256) void k_render_dense(DenseWorld w, ViewConsts v
   const bool active = px < v.W && py < v.H;
 
   Counters cnt = {0, 0, 0, 0, 0, 0, 0, 0};
-  DenseRay r;
-  L lk;
-  DenseHit h;
-  Primary p;
-  p.hit = false;
-  p.steps = 0;
-  float sox = 0.f, soy = 0.f, soz = 0.f;
-  int phase = 2;
-  bool term = true;
   if (active) {
+    DenseRay r;
+    L lk;
+    DenseHit h;
+    Primary p;
+    // ---- primary ray (primary.comp.glsl:31-43) ----
     float dx, dy, dz;
     raygen(v, px, py, dx, dy, dz);
-    // primary.comp.glsl:42-43
     float tn, tf;
     const float D = (float)w.dim;
     intersect_aabb(v.pos[0], v.pos[1], v.pos[2], dx, dy, dz, 0.0f, 0.0f, 0.0f, D, D, D, tn, tf);
@@ -196,43 +191,38 @@ __global__ __launch_bounds__(256) void k_render_dense(DenseWorld w, ViewConsts v
     float ex = (v.pos[0] + dx * tc) - 0.001f;
     float ey = (v.pos[1] + dy * tc) - 0.001f;
     float ez = (v.pos[2] + dz * tc) - 0.001f;
-    term = dense_begin<kStats>(r, lk, w, ex, ey, ez, dx, dy, dz, sc.max_primary, h, cnt.leaf,
-                               cnt.l1f, cnt.l0f);
-    phase = 0;
-  }
-  while (phase < 2) {
-    if (term) {
-      if (phase == 0) {
-        p.hit = h.hit;
-        p.steps = h.steps;
-        if (!h.hit) {
-          phase = 2;
-          break;
-        }
-        p.face = h.face;
-        p.voxel = (uint32_t)h.bx + w.dim * ((uint32_t)h.by + w.dim * (uint32_t)h.bz);
-        p.material = lk.material(w, h.bx, h.by, h.bz, h.v, cnt.leaf);
-        p.data = w.palette[p.material];
-        p.sub = h.sub;
-        p.hx = h.hx;
-        p.hy = h.hy;
-        p.hz = h.hz;
-        shadow_origin(p, sc.flags, sox, soy, soz);
-        phase = 1;
-        term = dense_begin<kStats>(r, lk, w, sox, soy, soz, sc.sun[0], sc.sun[1], sc.sun[2],
-                                   sc.max_shadow, h, cnt.leaf, cnt.l1f, cnt.l0f);
-        continue;
+    if (!dense_begin<kStats>(r, lk, w, ex, ey, ez, dx, dy, dz, sc.max_primary, h, cnt.l1f,
+                             cnt.l0f))
+      dense_run<kStats>(r, lk, w, h, cnt.l1f, cnt.l0f);
+    p.hit = h.hit;
+    p.steps = h.steps;
+    float sox = 0.f, soy = 0.f, soz = 0.f;
+    bool shit = false;
+    float shx = -1.0f, shy = -1.0f, shz = -1.0f;
+    int ssteps = 0;
+    if (p.hit) {
+      p.face = h.face;
+      p.voxel = (uint32_t)h.bx + w.dim * ((uint32_t)h.by + w.dim * (uint32_t)h.bz);
+      p.material = lk.material(w, h.code, h.v, cnt.leaf);
+      p.data = w.palette[p.material];
+      p.sub = h.sub;
+      p.hx = h.hx;
+      p.hy = h.hy;
+      p.hz = h.hz;
+      // ---- shadow ray (secondary.comp.glsl:25-42) ----
+      shadow_origin(p, sc.flags, sox, soy, soz);
+      if (!dense_begin<kStats>(r, lk, w, sox, soy, soz, sc.sun[0], sc.sun[1], sc.sun[2],
+                               sc.max_shadow, h, cnt.l1f, cnt.l0f))
+        dense_run<kStats>(r, lk, w, h, cnt.l1f, cnt.l0f);
+      shit = h.hit;
+      if (shit) {
+        shx = h.hx;
+        shy = h.hy;
+        shz = h.hz;
       }
-      phase = 2;
-      break;
+      ssteps = h.steps;
     }
-    term = dense_step<kStats>(r, lk, w, h, cnt.leaf, cnt.l1f, cnt.l0f);
-  }
-  if (active) {
-    bool shit = p.hit && h.hit;
-    float shx = shit ? h.hx : -1.0f, shy = shit ? h.hy : -1.0f, shz = shit ? h.hz : -1.0f;
-    write_pixel(v, sc, out, px, py, p, shit, shx, shy, shz, sox, soy, soz, p.hit ? h.steps : 0,
-                cnt);
+    write_pixel(v, sc, out, px, py, p, shit, shx, shy, shz, sox, soy, soz, ssteps, cnt);
   }
   if (kStats) flush_counters(out, cnt);
 }

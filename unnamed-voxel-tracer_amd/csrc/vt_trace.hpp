@@ -7,11 +7,12 @@
 // rounds once, so hits are bit-exact.
 //
 // Two traversal loops:
-//  * DenseTracer: worlds where one grid voxel is one reference block whose
-//    atlas model is a solid cube.  After the first lookup traceMap never leaves
-//    the block level (stepSize 3), so the state is kept in block units
-//    G = gridsCoords >> 3 with withinGridCoords still in x8 units; this is the
-//    same recurrence, with pos >> 3 == G + (int(within) >> 3).
+//  * dense_begin + dense_run: worlds where one grid voxel is one reference
+//    block whose atlas model is a solid cube.  After the first lookup
+//    traceMap never leaves the block level (stepSize 3), so the block
+//    coordinates G = gridsCoords >> 3 live in a Morton code while
+//    withinGridCoords stays in x8 units; this is the same recurrence, with
+//    pos >> 3 == G + (int(within) >> 3).
 //  * The literal general loop (trace_general) with both levels, used for the
 //    reference brickmap + atlas layout.
 #pragma once
@@ -155,86 +156,98 @@ __device__ __forceinline__ void sky(const float sun[3], float rx, float ry, floa
 }
 
 // ---------------------------------------------------------------------------
-// dense worlds
+// dense worlds: Morton-ordered voxels
 // ---------------------------------------------------------------------------
+// The material bytes are stored in Morton (Z) order: voxel (x,y,z) at
+// code = interleave(x, y, z) (x in bits 0,3,6,..., y in 1,4,..., z in 2,5,...).
+// A 128-byte cache line then holds a ~5^3 neighbourhood, a 4^3 brick is 64
+// consecutive codes and a 16^3 region 4096, whatever direction a ray moves.
+// The two occupancy levels fall out of the same order: L0 word code >> 6 has
+// bit code & 63, L1 word code >> 12 has bit (code >> 6) & 63.
 struct DenseWorld {
-  const uint8_t* vox;       // material bytes in 4^3 bricks (64 B each), bricks x-fastest
-  const uint64_t* l0;       // one bit per voxel of each 4^3 brick
-  const uint64_t* l1;       // one bit per brick of each 16^3 region
+  const uint8_t* vox;       // material byte of voxel (x,y,z) at morton(x,y,z)
+  const uint64_t* l0;       // 4^3 bricks: bit per voxel
+  const uint64_t* l1;       // 16^3 regions: bit per nonempty brick
   const uint32_t* palette;  // RGBA8 colour of material k (the solid atlas cube)
-  uint32_t dim;             // voxels per axis (MAP_DIMENSION)
-  uint32_t nb;              // bricks per axis (dim / 4)
-  uint32_t nr;              // regions per axis (dim / 16)
+  uint32_t dim;             // voxels per axis (MAP_DIMENSION), a power of two
+  uint32_t mx, my, mz;      // dilated masks of x, y, z over 3*log2(dim) bits
 };
 
-__device__ __forceinline__ size_t dense_byte_index(const DenseWorld& w, int bx, int by, int bz) {
-  uint32_t brick = ((uint32_t)(bz >> 2) * w.nb + (uint32_t)(by >> 2)) * w.nb + (uint32_t)(bx >> 2);
-  uint32_t local = (uint32_t)(bx & 3) | ((uint32_t)(by & 3) << 2) | ((uint32_t)(bz & 3) << 4);
-  return (size_t)brick * 64u + local;
+__device__ __forceinline__ uint32_t part1by2(uint32_t x) {
+  x &= 0x000003FFu;
+  x = (x | (x << 16)) & 0xFF0000FFu;
+  x = (x | (x << 8)) & 0x0300F00Fu;
+  x = (x | (x << 4)) & 0x030C30C3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+__device__ __forceinline__ uint32_t compact1by2(uint32_t x) {
+  x &= 0x09249249u;
+  x = (x | (x >> 2)) & 0x030C30C3u;
+  x = (x | (x >> 4)) & 0x0300F00Fu;
+  x = (x | (x >> 8)) & 0xFF0000FFu;
+  x = (x | (x >> 16)) & 0x000003FFu;
+  return x;
+}
+__device__ __forceinline__ uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+  return part1by2(x) | (part1by2(y) << 1) | (part1by2(z) << 2);
 }
 
 __device__ __forceinline__ bool in_dim(const DenseWorld& w, int bx, int by, int bz) {
   return ((uint32_t)bx < w.dim) & ((uint32_t)by < w.dim) & ((uint32_t)bz < w.dim);
 }
 
-// Occupancy lookup strategies (identical answers by contract).
+// Occupancy probe strategies (identical answers by contract).
 struct FlatLookup {
-  // returns the material byte (0 = empty); one gather per in-bounds lookup
+  // one byte gather per probe; returns the material (0 = empty)
   __device__ __forceinline__ void reset() {}
   template <bool kStats>
-  __device__ __forceinline__ uint32_t lookup(const DenseWorld& w, int bx, int by, int bz,
-                                             uint32_t& leaf, uint32_t&, uint32_t&) {
-    if (!in_dim(w, bx, by, bz)) return 0;
-    if (kStats) leaf++;
-    return w.vox[dense_byte_index(w, bx, by, bz)];
+  __device__ __forceinline__ uint32_t probe(const DenseWorld& w, uint32_t code, uint32_t&,
+                                            uint32_t&) {
+    return w.vox[code];
   }
-  __device__ __forceinline__ uint32_t material(const DenseWorld&, int, int, int, uint32_t v,
+  __device__ __forceinline__ uint32_t material(const DenseWorld&, uint32_t, uint32_t v,
                                                uint32_t&) {
     return v;
   }
 };
 
 struct BitmaskLookup {
-  uint32_t region_key, brick_key;
-  uint64_t region_word, brick_word;
+  // keeps the last brick word in registers; a probe in another brick fetches
+  // the region word when the 16^3 region changed, then the brick word only
+  // when the region word says the brick is not empty.
+  uint32_t brick, region;
+  uint64_t bword, rword;
   __device__ __forceinline__ void reset() {
-    region_key = 0xFFFFFFFFu;
-    brick_key = 0xFFFFFFFFu;
-    region_word = 0;
-    brick_word = 0;
+    brick = 0xFFFFFFFFu;
+    region = 0xFFFFFFFFu;
+    bword = 0;
+    rword = 0;
   }
-  // returns nonzero when the block is solid; fetches a region word when the
-  // 16^3 region changes and a brick word when the 4^3 brick changes and the
-  // region word says the brick is not empty.
   template <bool kStats>
-  __device__ __forceinline__ uint32_t lookup(const DenseWorld& w, int bx, int by, int bz,
-                                             uint32_t&, uint32_t& l1f, uint32_t& l0f) {
-    if (!in_dim(w, bx, by, bz)) return 0;
-    uint32_t kx = (uint32_t)bx >> 2, ky = (uint32_t)by >> 2, kz = (uint32_t)bz >> 2;
-    uint32_t bkey = kx | (ky << 10) | (kz << 20);
-    if (bkey != brick_key) {
-      brick_key = bkey;
-      uint32_t rkey = (kx >> 2) | ((ky >> 2) << 10) | ((kz >> 2) << 20);
-      if (rkey != region_key) {
-        region_key = rkey;
-        uint32_t ridx = ((kz >> 2) * w.nr + (ky >> 2)) * w.nr + (kx >> 2);
-        region_word = w.l1[ridx];
+  __device__ __forceinline__ uint32_t probe(const DenseWorld& w, uint32_t code, uint32_t& l1f,
+                                            uint32_t& l0f) {
+    const uint32_t bk = code >> 6;
+    if (bk != brick) {
+      brick = bk;
+      const uint32_t rk = code >> 12;
+      if (rk != region) {
+        region = rk;
+        rword = w.l1[rk];
         if (kStats) l1f++;
       }
-      uint32_t rbit = (kx & 3u) | ((ky & 3u) << 2) | ((kz & 3u) << 4);
-      brick_word = 0;
-      if ((region_word >> rbit) & 1ull) {
-        brick_word = w.l0[(kz * w.nb + ky) * w.nb + kx];
+      bword = 0;
+      if ((rword >> (bk & 63u)) & 1ull) {
+        bword = w.l0[bk];
         if (kStats) l0f++;
       }
     }
-    uint32_t vbit = (uint32_t)(bx & 3) | ((uint32_t)(by & 3) << 2) | ((uint32_t)(bz & 3) << 4);
-    return (uint32_t)(brick_word >> vbit) & 1u;
+    return (uint32_t)(bword >> (code & 63u)) & 1u;
   }
-  __device__ __forceinline__ uint32_t material(const DenseWorld& w, int bx, int by, int bz,
-                                               uint32_t, uint32_t& leaf) {
+  __device__ __forceinline__ uint32_t material(const DenseWorld& w, uint32_t code, uint32_t,
+                                               uint32_t& leaf) {
     leaf++;
-    return w.vox[dense_byte_index(w, bx, by, bz)];
+    return w.vox[code];
   }
 };
 
@@ -243,7 +256,8 @@ struct DenseHit {
   bool hit;
   int face;             // 1..6
   int bx, by, bz;       // block coordinates (pos >> 3)
-  uint32_t v;           // lookup value at the hit (flat: the material byte)
+  uint32_t code;        // morton(bx, by, bz)
+  uint32_t v;           // probe value at the hit (flat: the material byte)
   uint32_t sub;         // pos % 8 packed x + 8y + 64z
   float hx, hy, hz;     // hit_pos = gridsCoords + withinGridCoords
   int steps;
@@ -251,15 +265,15 @@ struct DenseHit {
 
 // Ray state at the block level.
 struct DenseRay {
-  int Gx, Gy, Gz;     // gridsCoords >> 3
   float wx, wy, wz;   // withinGridCoords, x8 units
   float dx, dy, dz;   // rayDir after the zero substitution
   float ix, iy, iz;   // 1 / rayDir
   float Px, Py, Pz;   // float(rayPositivity << 3)
   float Ax, Ay, Az;   // float((1 - rayPositivity) << 3) * 0.999f
-  int Sx, Sy, Sz;     // raySign
   int px, py, pz;     // rayPositivity
-  int m;              // minIdx
+  uint32_t c;         // Morton code of G = gridsCoords >> 3, axes the ray
+                      // travels negatively stored complemented
+  uint32_t flip;      // the complemented axes' masks; address = c ^ flip
   int steps, max_steps;
 };
 
@@ -269,20 +283,20 @@ struct DenseRay {
 template <bool kStats, class L>
 __device__ __forceinline__ bool dense_begin(DenseRay& r, L& lk, const DenseWorld& w, float ox,
                                             float oy, float oz, float dx, float dy, float dz,
-                                            int max_steps, DenseHit& h, uint32_t& leaf,
-                                            uint32_t& l1f, uint32_t& l0f) {
+                                            int max_steps, DenseHit& h, uint32_t& l1f,
+                                            uint32_t& l0f) {
   if (dx == 0.0f) dx = 0.001f;
   if (dy == 0.0f) dy = 0.001f;
   if (dz == 0.0f) dz = 0.001f;
   r.dx = dx;
   r.dy = dy;
   r.dz = dz;
-  r.Sx = dx > 0.0f ? 1 : (dx < 0.0f ? -1 : 0);
-  r.Sy = dy > 0.0f ? 1 : (dy < 0.0f ? -1 : 0);
-  r.Sz = dz > 0.0f ? 1 : (dz < 0.0f ? -1 : 0);
-  r.px = (1 + r.Sx) >> 1;
-  r.py = (1 + r.Sy) >> 1;
-  r.pz = (1 + r.Sz) >> 1;
+  const int sx = dx > 0.0f ? 1 : (dx < 0.0f ? -1 : 0);
+  const int sy = dy > 0.0f ? 1 : (dy < 0.0f ? -1 : 0);
+  const int sz = dz > 0.0f ? 1 : (dz < 0.0f ? -1 : 0);
+  r.px = (1 + sx) >> 1;
+  r.py = (1 + sy) >> 1;
+  r.pz = (1 + sz) >> 1;
   r.ix = 1.0f / dx;
   r.iy = 1.0f / dy;
   r.iz = 1.0f / dz;
@@ -292,14 +306,14 @@ __device__ __forceinline__ bool dense_begin(DenseRay& r, L& lk, const DenseWorld
   r.Ax = (float)((1 - r.px) << 3) * 0.999f;
   r.Ay = (float)((1 - r.py) << 3) * 0.999f;
   r.Az = (float)((1 - r.pz) << 3) * 0.999f;
-  r.m = 0;
+  r.flip = (r.px ? 0u : w.mx) | (r.py ? 0u : w.my) | (r.pz ? 0u : w.mz);
   r.steps = 0;
   r.max_steps = max_steps;
   lk.reset();
 
-  float sx = ox * 8.0f, sy = oy * 8.0f, sz = oz * 8.0f;
-  int gx = f2i_sat(sx), gy = f2i_sat(sy), gz = f2i_sat(sz);
-  float wx = sx - (float)gx, wy = sy - (float)gy, wz = sz - (float)gz;
+  float s8x = ox * 8.0f, s8y = oy * 8.0f, s8z = oz * 8.0f;
+  int gx = f2i_sat(s8x), gy = f2i_sat(s8y), gz = f2i_sat(s8z);
+  float wx = s8x - (float)gx, wy = s8y - (float)gy, wz = s8z - (float)gz;
 
   h.hit = false;
   h.steps = 0;
@@ -309,13 +323,19 @@ __device__ __forceinline__ bool dense_begin(DenseRay& r, L& lk, const DenseWorld
   // iteration 0, stepSize 0: pos = gridsCoords + int(within)
   int qx = gx + (int)wx, qy = gy + (int)wy, qz = gz + (int)wz;
   int bx = qx >> 3, by = qy >> 3, bz = qz >> 3;
-  uint32_t v = lk.template lookup<kStats>(w, bx, by, bz, leaf, l1f, l0f);
+  uint32_t v = 0;
+  uint32_t code = 0;
+  if (in_dim(w, bx, by, bz)) {
+    code = morton3((uint32_t)bx, (uint32_t)by, (uint32_t)bz);
+    v = lk.template probe<kStats>(w, code, l1f, l0f);
+  }
   if (v != 0u) {
     h.hit = true;
     h.face = 2 - r.px;  // minIdx == 0 on the first iteration
     h.bx = bx;
     h.by = by;
     h.bz = bz;
+    h.code = code;
     h.v = v;
     h.sub = (uint32_t)(qx & 7) | ((uint32_t)(qy & 7) << 3) | ((uint32_t)(qz & 7) << 6);
     h.hx = (float)gx + wx;
@@ -329,67 +349,91 @@ __device__ __forceinline__ bool dense_begin(DenseRay& r, L& lk, const DenseWorld
   r.wx = wx + (float)lx;
   r.wy = wy + (float)ly;
   r.wz = wz + (float)lz;
-  r.Gx = (gx - lx) >> 3;
-  r.Gy = (gy - ly) >> 3;
-  r.Gz = (gz - lz) >> 3;
+  r.c = morton3((uint32_t)(gx - lx) >> 3, (uint32_t)(gy - ly) >> 3, (uint32_t)(gz - lz) >> 3) ^
+        r.flip;
   return false;
 }
 
-// One DDA step (map.glsl:157-162) at the block level followed by the next
-// iteration's bounds check and lookup.  Returns true when the ray terminated.
+// Real-space increment of one axis of a Morton code (mask M); sets `out`
+// when the coordinate was already dim - 1.
+__device__ __forceinline__ uint32_t morton_inc(uint32_t a, uint32_t M, bool& out) {
+  out |= (a & M) == M;
+  return (((a | ~M) + 1u) & M) | (a & ~M);
+}
+
+// The block-level loop of traceMap (map.glsl:106-165 at stepSize 3) after
+// iteration 0.  A step is the DDA arithmetic (map.glsl:157-162), one dilated
+// increment of the crossed axis in the flipped Morton code and one probe:
+//   M = mask of the crossed axis,  c' = ((c & M) - M) & M | (c & ~M)
+// Leaving the grid is exactly the wrap of that increment, (c & M) == M,
+// which is traceMap's bounds test.  The probed block is pos >> 3 =
+// G + (int(within) >> 3): within reaches 8 on a positive axis the DDA did not
+// cross when two crossing times tie exactly (frequent for rays started on
+// lattice points, e.g. every shadow ray), so that correction is a short
+// branch inside the loop rather than an exit from it.
 template <bool kStats, class L>
-__device__ __forceinline__ bool dense_step(DenseRay& r, L& lk, const DenseWorld& w, DenseHit& h,
-                                           uint32_t& leaf, uint32_t& l1f, uint32_t& l0f) {
-  float tx = (r.Px - r.wx) * r.ix;
-  float ty = (r.Py - r.wy) * r.iy;
-  float tz = (r.Pz - r.wz) * r.iz;
-  bool xy = tx < ty, xz = tx < tz, yz = ty < tz;
-  bool m0 = xy & xz;
-  bool m1 = (!xy) & yz;
-  bool m2 = !(m0 | m1);
-  float tm = m0 ? tx : (m1 ? ty : tz);
-  float nwx = r.wx + r.dx * tm;
-  float nwy = r.wy + r.dy * tm;
-  float nwz = r.wz + r.dz * tm;
-  r.wx = m0 ? r.Ax : nwx;
-  r.wy = m1 ? r.Ay : nwy;
-  r.wz = m2 ? r.Az : nwz;
-  r.Gx += m0 ? r.Sx : 0;
-  r.Gy += m1 ? r.Sy : 0;
-  r.Gz += m2 ? r.Sz : 0;
-  r.m = m0 ? 0 : (m1 ? 1 : 2);
-  r.steps++;
-  if (r.steps >= r.max_steps) {
+__device__ __forceinline__ void dense_run(DenseRay& r, L& lk, const DenseWorld& w, DenseHit& h,
+                                          uint32_t& l1f, uint32_t& l0f) {
+  const float Px = r.Px, Py = r.Py, Pz = r.Pz, Ax = r.Ax, Ay = r.Ay, Az = r.Az;
+  const float dx = r.dx, dy = r.dy, dz = r.dz, ix = r.ix, iy = r.iy, iz = r.iz;
+  const uint32_t mx = w.mx, my = w.my, mz = w.mz, flip = r.flip;
+  const int max_steps = r.max_steps;
+  float wx = r.wx, wy = r.wy, wz = r.wz;
+  uint32_t c = r.c;
+  uint32_t M = mx;
+  uint32_t a;
+  int steps = r.steps;
+  uint32_t v;
+  bool wrapped;
+  do {
+    float tx = (Px - wx) * ix;
+    float ty = (Py - wy) * iy;
+    float tz = (Pz - wz) * iz;
+    bool xy = tx < ty, xz = tx < tz, yz = ty < tz;
+    bool m0 = xy & xz;
+    bool m1 = (!xy) & yz;
+    float tm = m0 ? tx : (m1 ? ty : tz);
+    float nwx = wx + dx * tm;
+    float nwy = wy + dy * tm;
+    float nwz = wz + dz * tm;
+    wx = m0 ? Ax : nwx;
+    wy = m1 ? Ay : nwy;
+    wz = (m0 | m1) ? nwz : Az;
+    M = m0 ? mx : (m1 ? my : mz);
+    const uint32_t cm = c & M;
+    wrapped = cm == M;
+    c = ((cm - M) & M) | (c & ~M);
+    steps++;
+    a = c ^ flip;
+    bool outside = false;
+    if (fmaxf(fmaxf(wx, wy), wz) >= 8.0f) {  // pos >> 3 = G + (int(within) >> 3)
+      if (wx >= 8.0f) a = morton_inc(a, mx, outside);
+      if (wy >= 8.0f) a = morton_inc(a, my, outside);
+      if (wz >= 8.0f) a = morton_inc(a, mz, outside);
+    }
+    v = outside ? 0u : lk.template probe<kStats>(w, a, l1f, l0f);
+  } while (!wrapped & (v == 0u) & (steps < max_steps));
+
+  if (steps >= max_steps || wrapped) {  // the loop cap, then the bounds test
     h.hit = false;
-    h.steps = r.steps;
-    return true;
+    h.steps = steps;
+    return;
   }
-  if (((uint32_t)r.Gx >= w.dim) | ((uint32_t)r.Gy >= w.dim) | ((uint32_t)r.Gz >= w.dim)) {
-    h.hit = false;
-    h.steps = r.steps;
-    return true;
-  }
-  // pos >> 3 with pos = 8*G + int(within): within lies in (-1, 9)
-  int bx = r.Gx + ((int)r.wx >> 3);
-  int by = r.Gy + ((int)r.wy >> 3);
-  int bz = r.Gz + ((int)r.wz >> 3);
-  uint32_t v = lk.template lookup<kStats>(w, bx, by, bz, leaf, l1f, l0f);
-  if (v != 0u) {
-    h.hit = true;
-    h.face = r.m == 0 ? 2 - r.px : (r.m == 1 ? 4 - r.py : 6 - r.pz);
-    h.bx = bx;
-    h.by = by;
-    h.bz = bz;
-    h.v = v;
-    h.sub = (uint32_t)((int)r.wx & 7) | ((uint32_t)((int)r.wy & 7) << 3) |
-            ((uint32_t)((int)r.wz & 7) << 6);
-    h.hx = (float)(r.Gx << 3) + r.wx;
-    h.hy = (float)(r.Gy << 3) + r.wy;
-    h.hz = (float)(r.Gz << 3) + r.wz;
-    h.steps = r.steps + 1;
-    return true;
-  }
-  return false;
+  const uint32_t g = c ^ flip;
+  const int m = M == mx ? 0 : (M == my ? 1 : 2);
+  h.hit = true;
+  h.face = m == 0 ? 2 - r.px : (m == 1 ? 4 - r.py : 6 - r.pz);
+  h.bx = (int)compact1by2(a);
+  h.by = (int)compact1by2(a >> 1);
+  h.bz = (int)compact1by2(a >> 2);
+  h.code = a;
+  h.v = v;
+  h.sub = (uint32_t)((int)wx & 7) | ((uint32_t)((int)wy & 7) << 3) |
+          ((uint32_t)((int)wz & 7) << 6);
+  h.hx = (float)(compact1by2(g) << 3) + wx;
+  h.hy = (float)(compact1by2(g >> 1) << 3) + wy;
+  h.hz = (float)(compact1by2(g >> 2) << 3) + wz;
+  h.steps = steps + 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -461,10 +505,8 @@ __device__ bool trace_general(const BrickmapWorld& w, float ox, float oy, float 
     int p[3] = {g[0] + (int)wc[0], g[1] + (int)wc[1], g[2] + (int)wc[2]};
     int b[3] = {p[0] >> 3, p[1] >> 3, p[2] >> 3};
     uint32_t block = bm_block(w, b[0], b[1], b[2]);
-    if (kStats) leaf++;
     if (block != 0u) {
       uint32_t sub = bm_sub(w, block, p[0] % 8, p[1] % 8, p[2] % 8);
-      if (kStats) leaf++;
       if (sub != 0u) {
         h.data = sub;
         h.face = m == 0 ? 2 - pv[0] : (m == 1 ? 4 - pv[1] : 6 - pv[2]);
