@@ -1,0 +1,67 @@
+"""Plants plausible mistakes in copies of the CPU restatement and checks that
+the pinning tests catch every one of them (the oracle is only as good as
+the checks that would notice it being wrong).  CPU only."""
+import ctypes as C
+import os
+import subprocess
+
+import pytest
+
+import oracle_ffi as O
+import test_oracle_pinning as P
+
+SRC = os.path.join(O.ORACLE_DIR, "vt_oracle.c")
+
+# (name, original text, mutated text, pinning checks expected to fail)
+MUTATIONS = [
+    ("anchor 0.999 -> 1.0", "<< s) * 0.999f;", "<< s) * 1.0f;",
+     [P.test_trace_matches_amanatides_woo_with_drift, P.test_empty_world_steps_and_exit]),
+    ("tie rule z>y>x -> x first", "m = t[0] < t[1] ? (t[0] < t[2] ? 0 : 2)",
+     "m = t[0] <= t[1] ? (t[0] <= t[2] ? 0 : 2)", [P.test_exact_tie_skips_the_corner_blocks]),
+    ("grid index transposed",
+     "return w->grid[(uint64_t)bx + (uint64_t)w->dim * ((uint64_t)by + (uint64_t)w->dim * bz)];",
+     "return w->grid[(uint64_t)bz + (uint64_t)w->dim * ((uint64_t)by + (uint64_t)w->dim * bx)];",
+     [P.test_trace_matches_bruteforce_first_hit]),
+    ("face sign flipped", "int face = m == 0 ? 2 - posv[0]", "int face = m == 0 ? 1 + posv[0]",
+     [P.test_first_hit_face_and_position, P.test_hit_face_is_the_entered_face]),
+    ("entry epsilon sign", "out[i] = (o[i] + d[i] * tc) - 0.001f;",
+     "out[i] = (o[i] + d[i] * tc) + 0.001f;", [P.test_entry_origin]),
+    ("pixel centre +0.5", "float u = (float)px / (float)W;", "float u = ((float)px + 0.5f) / (float)W;",
+     [P.test_raygen_matches_closed_form]),
+    ("aspect inverted", "v = v * ((float)H / (float)W);", "v = v * ((float)W / (float)H);",
+     [P.test_raygen_matches_closed_form]),
+    ("3-component normalize", "s = s + r[3] * r[3];", "s = s + 0.0f;",
+     [P.test_raygen_centre_pixel_four_component_normalize]),
+    ("entity t >= 0 check added", "if (tf >= tn && prevD >= tf) {",
+     "if (tf >= tn && tf >= 0.0f && prevD >= tf) {", [P.test_entities_closed_form]),
+    ("pool in-block index x+8z+64y",
+     "(uint64_t)(((bz % 8) * 8 + (by % 8)) * 8);", "(uint64_t)(((by % 8) * 8 + (bz % 8)) * 8);",
+     [P.test_brickmap_layout_renders_like_dense]),
+]
+
+
+@pytest.mark.parametrize("name,orig,mut,checks", MUTATIONS, ids=[m[0] for m in MUTATIONS])
+def test_pinning_catches_planted_mistake(tmp_path, name, orig, mut, checks):
+    text = open(SRC).read()
+    assert text.count(orig) == 1, f"mutation anchor not found once: {orig}"
+    (tmp_path / "vt_oracle.c").write_text(text.replace(orig, mut))
+    (tmp_path / "vt_oracle.h").write_text(open(os.path.join(O.ORACLE_DIR, "vt_oracle.h")).read())
+    so = tmp_path / "liboracle.so"
+    subprocess.run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-ffp-contract=off", "-o",
+                    str(so), str(tmp_path / "vt_oracle.c"), "-lm", "-lpthread"], check=True)
+    saved_dir, saved_lib = O.ORACLE_DIR, O._L
+    try:
+        O.ORACLE_DIR, O._L = str(tmp_path), None
+        caught = False
+        for check in checks:
+            try:
+                if "c0_render" in check.__code__.co_varnames[:check.__code__.co_argcount]:
+                    check(P.c0_render.__wrapped__())
+                else:
+                    check()
+            except AssertionError:
+                caught = True
+                break
+        assert caught, f"planted mistake '{name}' passed every pinning check"
+    finally:
+        O.ORACLE_DIR, O._L = saved_dir, saved_lib
