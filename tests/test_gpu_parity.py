@@ -107,3 +107,40 @@ def test_c1_rows_and_flat_equals_bitmask(ctx):
         o = O.render(ow, cam, W, H, 2048, 2048, S.SUN, rows=rows, gbuffer=True)
         compare(gf, o, rows=rows, stats=False, label=f"c1 rows {rows}")
     w.close()
+
+
+# ---------------------------------------------------------------------------
+# reference layout: chunk table + block pool + model atlas (drop-in path)
+# ---------------------------------------------------------------------------
+
+def test_brickmap_embedding_a_matches_oracle_and_dense(ctx, c0):
+    import brickmap_util as BU
+    grid, pal = c0
+    dim, chunks, pool, atlas = BU.embedding_a(grid, pal)
+    w = vt.World.brickmap(ctx, dim, chunks, pool, atlas)
+    cam, W, H = S.views("c0")[0]
+    g = gpu_render(ctx, w, cam, W, H, 512, 512)
+    o = O.render(O.BrickmapWorld(dim, chunks, pool, atlas), cam, W, H, 512, 512, S.SUN)
+    compare(g, o, stats=False, label="brickmap embedding A")
+    wd = vt.World.dense(ctx, grid, pal)
+    gd = gpu_render(ctx, wd, cam, W, H, 512, 512)
+    for k in ("hits", "steps", "position", "normal", "illumination"):
+        assert np.array_equal(g[k].view(np.uint32), gd[k].view(np.uint32)), k
+    w.close()
+    wd.close()
+
+
+@pytest.mark.parametrize("flags", [0, vt.VT_FLAG_GBUF_UNORM8])
+def test_brickmap_embedding_c_two_levels(ctx, c0, flags):
+    import brickmap_util as BU
+    grid, pal = c0
+    dim, chunks, pool, atlas = BU.embedding_c(grid, pal)
+    w = vt.World.brickmap(ctx, dim, chunks, pool, atlas)
+    cam = vt.look_at((8.0, 13.75, -2.5), (8.0, 8.0, 8.0), 70.0)
+    W = H = 256
+    g = gpu_render(ctx, w, cam, W, H, 1024, 256, flags=flags)
+    o = O.render(O.BrickmapWorld(dim, chunks, pool, atlas), cam, W, H, 1024, 256, S.SUN,
+                 flags=flags)
+    compare(g, o, stats=False, label=f"brickmap embedding C flags={flags}")
+    assert (g["hits"][..., 0] != 0xFFFFFFFF).mean() > 0.3
+    w.close()
