@@ -181,6 +181,53 @@ int vt_world_info(const vt_world* world, uint32_t* dim, size_t* device_bytes);
 int vt_render(vt_ctx* ctx, const vt_world* world, const vt_camera* camera,
               const vt_render_params* params, const vt_frame* frame, void* stream);
 
+/* ---- batched rendering --------------------------------------------------- */
+/* Many views of uneven resolution in ONE persistent launch: waves pull
+ * 64-pixel tiles of all views from per-XCD work queues.  No reference
+ * counterpart (the engine renders one view per frame, game.zig:232-256); it
+ * runs the same per-pixel path as vt_render, so every view's outputs are
+ * bit-identical to a vt_render of that view.  Dense worlds only.           */
+typedef struct vt_view {
+  vt_camera camera;
+  uint32_t width, height;
+  uint32_t _pad[2];
+  vt_frame frame; /* this view's outputs; frame.stats is ignored */
+} vt_view;
+
+/* wave tile shapes (log2 of the tile width; 64 pixels per tile) */
+#define VT_TILE_8X8 3u
+#define VT_TILE_16X4 4u
+#define VT_TILE_32X2 5u
+
+typedef struct vt_tuning {
+  uint32_t tile;          /* VT_TILE_*                                      */
+  uint32_t chunk;         /* tiles a wave takes per queue grab, 1..64       */
+  uint32_t traversal;     /* VT_TRAVERSAL_* (overrides params->traversal)   */
+  uint32_t blocks_per_cu; /* persistent 256-thread blocks per CU, 1..16     */
+} vt_tuning;
+
+typedef struct vt_batch vt_batch;
+/* Copies the view table to the device (a fixed argument buffer, so the
+ * launch can be captured in a graph and replayed).  params->width/height are
+ * ignored (each view has its own); stats may be NULL (device uint64[8]).    */
+int vt_batch_create(vt_ctx* ctx, const vt_world* world, const vt_view* views, uint32_t n_views,
+                    const vt_render_params* params, const vt_tuning* tuning,
+                    unsigned long long* stats, vt_batch** out);
+/* Enqueues the queue reset and the persistent launch on `stream`. */
+int vt_batch_launch(vt_batch* batch, void* stream);
+/* Captures reset + launch into a hipGraph (on a stream the library owns);
+ * vt_batch_graph_launch then replays it on any stream.  Calls that overlap
+ * on one batch race on its queue counters. */
+int vt_batch_capture_graph(vt_batch* batch);
+int vt_batch_graph_launch(vt_batch* batch, void* stream);
+int vt_batch_destroy(vt_batch* batch);
+/* The autotuner: times every tile shape x chunk {1,2,4,8} x traversal
+ * (flat, bitmask) on these views (`reps` timed launches each after one
+ * warm-up) and returns the fastest tuning and its mean launch time. */
+int vt_batch_autotune(vt_ctx* ctx, const vt_world* world, const vt_view* views,
+                      uint32_t n_views, const vt_render_params* params, int reps,
+                      vt_tuning* best, float* best_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,118 @@
+"""Batched persistent rendering (vt_batch_*) vs one vt_render per view.
+
+The batch runs the same per-pixel path through a work queue, so every view's
+outputs must be bit-identical to a plain render of that view, for every tile
+shape, queue chunk and traversal; a captured hipGraph must reproduce the
+eager launch on every replay (with other work and synchronisations between
+replays)."""
+import numpy as np
+import pytest
+
+import vt
+from vt import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(256, 256), (123, 77), (8, 8), (3, 5), (300, 17), (64, 200)]
+
+
+def make_views():
+    rng = np.random.default_rng(11)
+    out = [S.views("c0")[0][0]]
+    for _ in range(len(SIZES) - 1):
+        eye = rng.uniform(-30, 160, 3)
+        eye[1] = rng.uniform(90, 140)
+        out.append(vt.look_at(eye, rng.uniform(20, 108, 3), 70.0))
+    return out
+
+
+@pytest.fixture(scope="module")
+def setup(ctx):
+    import torch
+    grid = S.generate(S.spec("c0"))
+    world = vt.World.dense(ctx, grid, S.palette())
+    cams = make_views()
+    prm = vt.params(0, 0, 512, 512)
+    ref = []
+    for cam, (W, H) in zip(cams, SIZES):
+        f = vt.Frame.alloc(W, H, steps=True, stats=True)
+        vt.render(ctx, world, cam, f, vt.params(W, H, 512, 512))
+        ref.append(f)
+    torch.cuda.synchronize()
+    yield world, cams, prm, ref
+    world.close()
+
+
+def fresh_frames():
+    fs = [vt.Frame.alloc(W, H, steps=True) for (W, H) in SIZES]
+    for f in fs:
+        f.shaded.fill_(float("nan"))
+        f.hits.fill_(-7)
+        f.steps.fill_(-7)
+    return fs
+
+
+def assert_same(frames, ref, label):
+    for i, (f, r) in enumerate(zip(frames, ref)):
+        for k in ("hits", "steps", "shaded"):
+            a = getattr(f, k).cpu().numpy().view(np.uint32)
+            b = getattr(r, k).cpu().numpy().view(np.uint32)
+            assert np.array_equal(a, b), f"{label}: view {i} {SIZES[i]} {k} differs"
+
+
+@pytest.mark.parametrize("tile", [vt.VT_TILE_8X8, vt.VT_TILE_16X4, vt.VT_TILE_32X2])
+@pytest.mark.parametrize("chunk", [1, 4])
+@pytest.mark.parametrize("traversal", [vt.VT_TRAVERSAL_FLAT, vt.VT_TRAVERSAL_BITMASK])
+def test_batch_matches_per_view_render(ctx, setup, tile, chunk, traversal):
+    import torch
+    world, cams, prm, ref = setup
+    frames = fresh_frames()
+    b = vt.Batch(ctx, world, list(zip(cams, frames)), prm, vt.tuning(tile, chunk, traversal))
+    b.launch()
+    torch.cuda.synchronize()
+    print(f"[batch] tile={tile} chunk={chunk} traversal={traversal}", flush=True)
+    assert_same(frames, ref, f"tile={tile} chunk={chunk} trav={traversal}")
+    b.close()
+
+
+def test_batch_stats_sum_views(ctx, setup):
+    import torch
+    world, cams, prm, ref = setup
+    stats = torch.zeros(8, dtype=torch.int64, device="cuda")
+    frames = fresh_frames()
+    b = vt.Batch(ctx, world, list(zip(cams, frames)), prm, vt.tuning(), stats=stats)
+    b.launch()
+    torch.cuda.synchronize()
+    total = sum(r.stats.cpu().numpy() for r in ref)
+    assert np.array_equal(stats.cpu().numpy()[[0, 1, 2, 3, 7]], total[[0, 1, 2, 3, 7]])
+    b.close()
+
+
+def test_batch_graph_replays(ctx, setup):
+    import torch
+    world, cams, prm, ref = setup
+    frames = fresh_frames()
+    b = vt.Batch(ctx, world, list(zip(cams, frames)), prm, vt.tuning(vt.VT_TILE_16X4, 2))
+    b.capture_graph()
+    side = vt.Frame.alloc(64, 64)
+    for rep in range(4):
+        for f in frames:  # clear the outputs with the program's own eager work
+            f.shaded.fill_(float("nan"))
+            f.hits.fill_(-7)
+            f.steps.fill_(-7)
+        vt.render(ctx, world, cams[1], side, vt.params(64, 64, 512, 512))
+        torch.cuda.synchronize()
+        b.graph_launch()
+        torch.cuda.synchronize()
+        print(f"[batch] graph replay {rep}", flush=True)
+        assert_same(frames, ref, f"graph replay {rep}")
+    b.close()
+
+
+def test_autotune_picks_a_valid_tuning(ctx, setup):
+    world, cams, prm, ref = setup
+    frames = fresh_frames()
+    best, ms = vt.autotune(ctx, world, list(zip(cams, frames)), prm, reps=1)
+    assert best.tile in (3, 4, 5) and best.chunk in (1, 2, 4, 8)
+    assert best.traversal in (0, 1) and best.blocks_per_cu == 8
+    assert 0.0 < ms < 1000.0

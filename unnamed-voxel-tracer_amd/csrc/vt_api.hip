@@ -12,10 +12,11 @@
 #include "vt_internal.hpp"
 
 namespace {
-
 thread_local char g_detail[512] = "";
+}  // namespace
 
-int fail(int status, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+namespace vt {
+
 int fail(int status, const char* fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
@@ -28,17 +29,19 @@ int hip_fail(hipError_t e, const char* what) {
   return fail(VT_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
 }
 
-#define VT_HIP(call)                                   \
-  do {                                                 \
-    hipError_t e_ = (call);                            \
-    if (e_ != hipSuccess) return hip_fail(e_, #call);  \
-  } while (0)
-
 bool aligned(const void* p, size_t a) { return ((uintptr_t)p & (a - 1)) == 0; }
 
 void dev_free(void* p) {
   if (p) (void)hipFree(p);
 }
+
+}  // namespace vt
+
+namespace {
+using vt::aligned;
+using vt::dev_free;
+using vt::fail;
+using vt::hip_fail;
 
 void free_world(vt_world* w) {
   if (!w) return;
@@ -101,6 +104,79 @@ BrickmapWorld brickmap_view(const vt_world* w) {
   b.atlas = w->atlas;
   b.dim = w->dim;
   return b;
+}
+
+}  // namespace vt
+
+namespace vt {
+
+int check_image_size(uint32_t W, uint32_t H, const char* who) {
+  if (W == 0 || H == 0 || W > 16384 || H > 16384)
+    return fail(VT_ERR_INVALID_ARGUMENT, "%s: image %ux%u outside [1,16384]^2", who, W, H);
+  return VT_OK;
+}
+
+int check_params(const vt_render_params& p, const vt_world* world, const char* who) {
+  if (p.max_steps_primary < 0 || p.max_steps_primary > 65535 || p.max_steps_shadow < 0 ||
+      p.max_steps_shadow > 65535)
+    return fail(VT_ERR_INVALID_ARGUMENT, "%s: step caps must lie in [0, 65535]", who);
+  if (p.flags & ~(VT_FLAG_GBUF_UNORM8 | VT_FLAG_LAMBERT))
+    return fail(VT_ERR_INVALID_ARGUMENT, "%s: unknown flags 0x%x", who, p.flags);
+  if (p.traversal > VT_TRAVERSAL_BITMASK)
+    return fail(VT_ERR_INVALID_ARGUMENT, "%s: unknown traversal %u", who, p.traversal);
+  if (world->kind == WorldKind::Brickmap && p.traversal != VT_TRAVERSAL_FLAT)
+    return fail(VT_ERR_UNSUPPORTED, "%s: brickmap worlds trace with VT_TRAVERSAL_FLAT", who);
+  if ((p.flags & VT_FLAG_LAMBERT) && (!p.albedo || world->kind != WorldKind::Dense))
+    return fail(VT_ERR_UNSUPPORTED, "%s: VT_FLAG_LAMBERT needs a dense world and albedo", who);
+  if (p.albedo && !aligned(p.albedo, 4)) return fail(VT_ERR_MISALIGNED, "%s: albedo", who);
+  for (int i = 0; i < 3; i++)
+    if (!std::isfinite(p.sun[i])) return fail(VT_ERR_INVALID_ARGUMENT, "%s: sun not finite", who);
+  return VT_OK;
+}
+
+int check_frame(const vt_frame* frame, const char* who) {
+  const void* f16[] = {frame->shaded, frame->color, frame->normal, frame->position,
+                       frame->illumination};
+  for (const void* q : f16)
+    if (q && !aligned(q, 16)) return fail(VT_ERR_MISALIGNED, "%s: image not 16B aligned", who);
+  if ((frame->hits && !aligned(frame->hits, 4)) || (frame->steps && !aligned(frame->steps, 4)) ||
+      (frame->stats && !aligned(frame->stats, 8)))
+    return fail(VT_ERR_MISALIGNED, "%s: hits/steps/stats misaligned", who);
+  return VT_OK;
+}
+
+ViewConsts view_consts(const vt_camera* camera, uint32_t W, uint32_t H) {
+  ViewConsts v;
+  std::memcpy(v.pos, camera->position, sizeof v.pos);
+  std::memcpy(v.view, camera->view, sizeof v.view);
+  v.tan_half = (float)std::tan((double)(camera->fov / 2.0f));
+  v.W = W;
+  v.H = H;
+  return v;
+}
+
+ShadeConsts shade_consts(const vt_render_params& p) {
+  ShadeConsts sc;
+  std::memcpy(sc.sun, p.sun, sizeof sc.sun);
+  sc.max_primary = p.max_steps_primary;
+  sc.max_shadow = p.max_steps_shadow;
+  sc.flags = p.flags;
+  sc.ambient = p.ambient;
+  sc.albedo = p.albedo;
+  return sc;
+}
+
+FrameOut frame_out(const vt_frame* frame) {
+  FrameOut o;
+  o.shaded = reinterpret_cast<float4*>(frame->shaded);
+  o.hits = reinterpret_cast<uint32_t*>(frame->hits);
+  o.color = reinterpret_cast<float4*>(frame->color);
+  o.normal = reinterpret_cast<float4*>(frame->normal);
+  o.position = reinterpret_cast<float4*>(frame->position);
+  o.illum = reinterpret_cast<float4*>(frame->illumination);
+  o.steps = frame->steps;
+  o.stats = frame->stats;
+  return o;
 }
 
 }  // namespace vt
@@ -279,54 +355,14 @@ int vt_render(vt_ctx* ctx, const vt_world* world, const vt_camera* camera,
     return fail(VT_ERR_NULL_POINTER, "vt_render: ctx, world, camera, params, frame required");
   if (world->ctx != ctx) return fail(VT_ERR_WRONG_CONTEXT, "vt_render: world from another ctx");
   const vt_render_params& p = *params;
-  if (p.width == 0 || p.height == 0 || p.width > 16384 || p.height > 16384)
-    return fail(VT_ERR_INVALID_ARGUMENT, "vt_render: image %ux%u outside [1,16384]^2", p.width,
-                p.height);
-  if (p.max_steps_primary < 0 || p.max_steps_primary > 65535 || p.max_steps_shadow < 0 ||
-      p.max_steps_shadow > 65535)
-    return fail(VT_ERR_INVALID_ARGUMENT, "vt_render: step caps must lie in [0, 65535]");
-  if (p.flags & ~(VT_FLAG_GBUF_UNORM8 | VT_FLAG_LAMBERT))
-    return fail(VT_ERR_INVALID_ARGUMENT, "vt_render: unknown flags 0x%x", p.flags);
-  if (p.traversal > VT_TRAVERSAL_BITMASK)
-    return fail(VT_ERR_INVALID_ARGUMENT, "vt_render: unknown traversal %u", p.traversal);
-  if (world->kind == WorldKind::Brickmap && p.traversal != VT_TRAVERSAL_FLAT)
-    return fail(VT_ERR_UNSUPPORTED, "vt_render: brickmap worlds trace with VT_TRAVERSAL_FLAT");
-  if ((p.flags & VT_FLAG_LAMBERT) && (!p.albedo || world->kind != WorldKind::Dense))
-    return fail(VT_ERR_UNSUPPORTED, "vt_render: VT_FLAG_LAMBERT needs a dense world and albedo");
-  if (p.albedo && !aligned(p.albedo, 4)) return fail(VT_ERR_MISALIGNED, "vt_render: albedo");
-  const void* f16[] = {frame->shaded, frame->color, frame->normal, frame->position,
-                       frame->illumination};
-  for (const void* q : f16)
-    if (q && !aligned(q, 16)) return fail(VT_ERR_MISALIGNED, "vt_render: image not 16B aligned");
-  if ((frame->hits && !aligned(frame->hits, 4)) || (frame->steps && !aligned(frame->steps, 4)) ||
-      (frame->stats && !aligned(frame->stats, 8)))
-    return fail(VT_ERR_MISALIGNED, "vt_render: hits/steps/stats misaligned");
-  for (int i = 0; i < 3; i++)
-    if (!std::isfinite(p.sun[i])) return fail(VT_ERR_INVALID_ARGUMENT, "vt_render: sun not finite");
-
+  int st = vt::check_image_size(p.width, p.height, "vt_render");
+  if (st == VT_OK) st = vt::check_params(p, world, "vt_render");
+  if (st == VT_OK) st = vt::check_frame(frame, "vt_render");
+  if (st != VT_OK) return st;
   VT_HIP(hipSetDevice(ctx->device));
-  vt::ViewConsts v;
-  std::memcpy(v.pos, camera->position, sizeof v.pos);
-  std::memcpy(v.view, camera->view, sizeof v.view);
-  v.tan_half = (float)std::tan((double)(camera->fov / 2.0f));
-  v.W = p.width;
-  v.H = p.height;
-  vt::ShadeConsts sc;
-  std::memcpy(sc.sun, p.sun, sizeof sc.sun);
-  sc.max_primary = p.max_steps_primary;
-  sc.max_shadow = p.max_steps_shadow;
-  sc.flags = p.flags;
-  sc.ambient = p.ambient;
-  sc.albedo = p.albedo;
-  vt::FrameOut o;
-  o.shaded = reinterpret_cast<float4*>(frame->shaded);
-  o.hits = reinterpret_cast<uint32_t*>(frame->hits);
-  o.color = reinterpret_cast<float4*>(frame->color);
-  o.normal = reinterpret_cast<float4*>(frame->normal);
-  o.position = reinterpret_cast<float4*>(frame->position);
-  o.illum = reinterpret_cast<float4*>(frame->illumination);
-  o.steps = frame->steps;
-  o.stats = frame->stats;
+  const vt::ViewConsts v = vt::view_consts(camera, p.width, p.height);
+  const vt::ShadeConsts sc = vt::shade_consts(p);
+  const vt::FrameOut o = vt::frame_out(frame);
   vt::TileShape tile{3};
   hipError_t e;
   if (world->kind == WorldKind::Dense)

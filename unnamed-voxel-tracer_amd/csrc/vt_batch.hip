@@ -1,0 +1,301 @@
+// vt_batch.hip -- many views of uneven resolution in one persistent launch.
+//
+// Work unit: a 64-pixel tile (one wave).  The tiles of all views are numbered
+// view by view (tile_base = prefix sum of tiles per view) and split into 8
+// contiguous ranges, one per XCD: a block dealt to XCD x (blocks are dealt
+// round-robin, x = blockIdx % 8) pulls `chunk` tiles at a time from queue x
+// with one atomic, so neighbouring tiles -- which touch neighbouring voxels --
+// stay in one XCD's L2; when its queue runs dry a wave steals from the other
+// queues in order.  Every wave reaches the end of all 8 queues, so the grid
+// drains.  The per-pixel work is trace_pixel_dense, shared with vt_render.
+#include <chrono>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "vt_pixel.hpp"
+
+namespace vt {
+
+struct BatchView {
+  ViewConsts v;
+  FrameOut out;
+  uint32_t tiles_x;
+  uint32_t tile_base;
+};
+
+constexpr uint32_t kQueues = 8;
+
+__global__ void k_batch_reset(uint32_t* __restrict__ queues) {
+  if (threadIdx.x < kQueues) queues[threadIdx.x] = 0u;
+}
+
+template <class L, bool kStats>
+__global__ __launch_bounds__(256) void k_render_batch(DenseWorld w, ShadeConsts sc,
+                                                      const BatchView* __restrict__ views,
+                                                      uint32_t n_views, uint32_t total_tiles,
+                                                      uint32_t* __restrict__ queues,
+                                                      uint32_t chunk, uint32_t log2_tw,
+                                                      unsigned long long* stats) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t tw = 1u << log2_tw, th = 64u >> log2_tw;
+  const uint32_t home = blockIdx.x % kQueues;
+  Counters cnt = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t k = 0; k < kQueues; k++) {
+    const uint32_t q = (home + k) % kQueues;
+    const uint32_t lo = (uint32_t)(((uint64_t)total_tiles * q) / kQueues);
+    const uint32_t hi = (uint32_t)(((uint64_t)total_tiles * (q + 1)) / kQueues);
+    for (;;) {
+      uint32_t got = 0;
+      if (lane == 0) got = atomicAdd(queues + q, chunk);
+      got = __builtin_amdgcn_readfirstlane(__shfl(got, 0, 64));
+      if (got >= hi - lo) break;
+      const uint32_t t1 = lo + min(got + chunk, hi - lo);
+      for (uint32_t t = lo + got; t < t1; t++) {
+        // the view owning tile t: last view whose tile_base <= t
+        uint32_t a = 0, b = n_views;
+        while (b - a > 1u) {
+          const uint32_t m = (a + b) >> 1;
+          if (views[m].tile_base <= t) a = m;
+          else b = m;
+        }
+        a = __builtin_amdgcn_readfirstlane(a);
+        const BatchView& bv = views[a];
+        const uint32_t local = t - bv.tile_base;
+        const uint32_t ty = local / bv.tiles_x, tx = local - ty * bv.tiles_x;
+        const uint32_t px = tx * tw + (lane & (tw - 1u));
+        const uint32_t py = ty * th + (lane >> log2_tw);
+        const ViewConsts v = bv.v;
+        if (px < v.W && py < v.H) trace_pixel_dense<L, kStats>(w, v, sc, bv.out, px, py, cnt);
+      }
+    }
+  }
+  if (kStats) {
+    FrameOut o;
+    o.stats = stats;
+    flush_counters(o, cnt);
+  }
+}
+
+}  // namespace vt
+
+struct vt_batch {
+  vt_ctx* ctx;
+  const vt_world* world;
+  uint32_t n_views, total_tiles, nblocks;
+  vt_tuning tuning;
+  vt::ShadeConsts sc;
+  vt::BatchView* d_views;
+  uint32_t* d_queues;
+  unsigned long long* stats;
+  hipStream_t cap_stream;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+namespace {
+
+using vt::fail;
+using vt::hip_fail;
+
+hipError_t enqueue(const vt_batch* b, hipStream_t s) {
+  const vt::DenseWorld w = vt::dense_view(b->world);
+  hipLaunchKernelGGL(vt::k_batch_reset, dim3(1), dim3(64), 0, s, b->d_queues);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const dim3 grid(b->nblocks), block(256);
+  const uint32_t tile = b->tuning.tile, chunk = b->tuning.chunk;
+#define VT_BATCH_LAUNCH(L, S)                                                                  \
+  hipLaunchKernelGGL((vt::k_render_batch<vt::L, S>), grid, block, 0, s, w, b->sc, b->d_views, \
+                     b->n_views, b->total_tiles, b->d_queues, chunk, tile, b->stats)
+  if (b->tuning.traversal == VT_TRAVERSAL_BITMASK) {
+    if (b->stats) VT_BATCH_LAUNCH(BitmaskLookup, true);
+    else VT_BATCH_LAUNCH(BitmaskLookup, false);
+  } else {
+    if (b->stats) VT_BATCH_LAUNCH(FlatLookup, true);
+    else VT_BATCH_LAUNCH(FlatLookup, false);
+  }
+#undef VT_BATCH_LAUNCH
+  return hipGetLastError();
+}
+
+void release(vt_batch* b) {
+  if (!b) return;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+  vt::dev_free(b->d_views);
+  vt::dev_free(b->d_queues);
+  delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vt_batch_create(vt_ctx* ctx, const vt_world* world, const vt_view* views, uint32_t n_views,
+                    const vt_render_params* params, const vt_tuning* tuning,
+                    unsigned long long* stats, vt_batch** out) {
+  if (!out) return fail(VT_ERR_NULL_POINTER, "vt_batch_create: out is NULL");
+  *out = nullptr;
+  if (!ctx || !world || !views || !params || !tuning)
+    return fail(VT_ERR_NULL_POINTER, "vt_batch_create: ctx, world, views, params, tuning required");
+  if (world->ctx != ctx) return fail(VT_ERR_WRONG_CONTEXT, "vt_batch_create: world from another ctx");
+  if (world->kind != WorldKind::Dense)
+    return fail(VT_ERR_UNSUPPORTED, "vt_batch_create: batched rendering needs a dense world");
+  if (n_views == 0 || n_views > 65536)
+    return fail(VT_ERR_INVALID_ARGUMENT, "vt_batch_create: n_views %u outside [1, 65536]", n_views);
+  const vt_tuning t = *tuning;
+  if (t.tile < VT_TILE_8X8 || t.tile > VT_TILE_32X2 || t.chunk < 1 || t.chunk > 64 ||
+      t.traversal > VT_TRAVERSAL_BITMASK || t.blocks_per_cu < 1 || t.blocks_per_cu > 16)
+    return fail(VT_ERR_INVALID_ARGUMENT, "vt_batch_create: tuning out of range");
+  if (stats && !vt::aligned(stats, 8)) return fail(VT_ERR_MISALIGNED, "vt_batch_create: stats");
+  vt_render_params p = *params;
+  p.traversal = t.traversal;
+  int st = vt::check_params(p, world, "vt_batch_create");
+  if (st != VT_OK) return st;
+  const uint32_t tw = 1u << t.tile, th = 64u >> t.tile;
+  std::vector<vt::BatchView> table(n_views);
+  uint64_t tiles = 0;
+  for (uint32_t i = 0; i < n_views; i++) {
+    const vt_view& v = views[i];
+    if ((st = vt::check_image_size(v.width, v.height, "vt_batch_create")) != VT_OK ||
+        (st = vt::check_frame(&v.frame, "vt_batch_create")) != VT_OK)
+      return st;
+    vt::BatchView& b = table[i];
+    std::memset(&b, 0, sizeof b);
+    b.v = vt::view_consts(&v.camera, v.width, v.height);
+    b.out = vt::frame_out(&v.frame);
+    b.out.stats = nullptr;
+    b.tiles_x = (v.width + tw - 1) / tw;
+    b.tile_base = (uint32_t)tiles;
+    tiles += (uint64_t)b.tiles_x * ((v.height + th - 1) / th);
+    if (tiles >= (1ull << 31))
+      return fail(VT_ERR_INVALID_ARGUMENT, "vt_batch_create: more than 2^31 tiles");
+  }
+  VT_HIP(hipSetDevice(ctx->device));
+  vt_batch* b = new (std::nothrow) vt_batch;
+  if (!b) return fail(VT_ERR_OUT_OF_MEMORY, "vt_batch_create: host allocation");
+  std::memset(b, 0, sizeof *b);
+  b->ctx = ctx;
+  b->world = world;
+  b->n_views = n_views;
+  b->total_tiles = (uint32_t)tiles;
+  b->tuning = t;
+  b->sc = vt::shade_consts(p);
+  b->stats = stats;
+  b->nblocks = (uint32_t)ctx->compute_units * t.blocks_per_cu;
+  hipError_t e = hipMalloc((void**)&b->d_views, sizeof(vt::BatchView) * n_views);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_queues, sizeof(uint32_t) * vt::kQueues);
+  if (e == hipSuccess)
+    e = hipMemcpy(b->d_views, table.data(), sizeof(vt::BatchView) * n_views,
+                  hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    release(b);
+    return hip_fail(e, "vt_batch_create: device table");
+  }
+  *out = b;
+  return VT_OK;
+}
+
+int vt_batch_launch(vt_batch* batch, void* stream) {
+  if (!batch) return fail(VT_ERR_NULL_POINTER, "vt_batch_launch: batch is NULL");
+  VT_HIP(hipSetDevice(batch->ctx->device));
+  hipError_t e = enqueue(batch, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "vt_batch_launch");
+  return VT_OK;
+}
+
+int vt_batch_capture_graph(vt_batch* b) {
+  if (!b) return fail(VT_ERR_NULL_POINTER, "vt_batch_capture_graph: batch is NULL");
+  VT_HIP(hipSetDevice(b->ctx->device));
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  if (!b->cap_stream) VT_HIP(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+  VT_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = enqueue(b, b->cap_stream);
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(b->cap_stream, &g);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return hip_fail(e != hipSuccess ? e : e2, "vt_batch_capture_graph: capture");
+  }
+  b->graph = g;
+  VT_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  return VT_OK;
+}
+
+int vt_batch_graph_launch(vt_batch* b, void* stream) {
+  if (!b) return fail(VT_ERR_NULL_POINTER, "vt_batch_graph_launch: batch is NULL");
+  if (!b->exec) return fail(VT_ERR_INVALID_ARGUMENT, "vt_batch_graph_launch: no captured graph");
+  VT_HIP(hipSetDevice(b->ctx->device));
+  VT_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
+  return VT_OK;
+}
+
+int vt_batch_destroy(vt_batch* b) {
+  if (!b) return fail(VT_ERR_NULL_POINTER, "vt_batch_destroy: batch is NULL");
+  (void)hipSetDevice(b->ctx->device);
+  release(b);
+  return VT_OK;
+}
+
+int vt_batch_autotune(vt_ctx* ctx, const vt_world* world, const vt_view* views,
+                      uint32_t n_views, const vt_render_params* params, int reps,
+                      vt_tuning* best, float* best_ms) {
+  if (!best || !best_ms) return fail(VT_ERR_NULL_POINTER, "vt_batch_autotune: outputs required");
+  if (!ctx) return fail(VT_ERR_NULL_POINTER, "vt_batch_autotune: ctx is NULL");
+  if (reps < 1 || reps > 1000)
+    return fail(VT_ERR_INVALID_ARGUMENT, "vt_batch_autotune: reps %d outside [1, 1000]", reps);
+  VT_HIP(hipSetDevice(ctx->device));
+  hipStream_t s;
+  VT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  float best_t = 1e30f;
+  vt_tuning best_cfg = {VT_TILE_8X8, 1, VT_TRAVERSAL_FLAT, 8};
+  int st = VT_OK;
+  const uint32_t tiles[] = {VT_TILE_8X8, VT_TILE_16X4, VT_TILE_32X2};
+  const uint32_t chunks[] = {1, 2, 4, 8};
+  const uint32_t travs[] = {VT_TRAVERSAL_FLAT, VT_TRAVERSAL_BITMASK};
+  for (uint32_t tr : travs)
+    for (uint32_t ti : tiles)
+      for (uint32_t ch : chunks) {
+        vt_tuning cfg = {ti, ch, tr, 8};
+        vt_batch* b = nullptr;
+        st = vt_batch_create(ctx, world, views, n_views, params, &cfg, nullptr, &b);
+        if (st != VT_OK) goto done;
+        st = vt_batch_launch(b, s);  // warm-up
+        if (st == VT_OK && hipEventRecord(e0, s) != hipSuccess) st = VT_ERR_DEVICE;
+        for (int r = 0; st == VT_OK && r < reps; r++) st = vt_batch_launch(b, s);
+        if (st == VT_OK && hipEventRecord(e1, s) != hipSuccess) st = VT_ERR_DEVICE;
+        if (st == VT_OK && hipEventSynchronize(e1) != hipSuccess) st = VT_ERR_DEVICE;
+        float ms = 0.f;
+        if (st == VT_OK) (void)hipEventElapsedTime(&ms, e0, e1);
+        vt_batch_destroy(b);
+        if (st != VT_OK) goto done;
+        ms /= (float)reps;
+        if (ms < best_t) {
+          best_t = ms;
+          best_cfg = cfg;
+        }
+      }
+done:
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  if (st != VT_OK) return st == VT_ERR_DEVICE ? fail(st, "vt_batch_autotune: timing") : st;
+  *best = best_cfg;
+  *best_ms = best_t;
+  return VT_OK;
+}
+
+}  // extern "C"

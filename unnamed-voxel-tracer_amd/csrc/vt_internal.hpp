@@ -36,6 +36,19 @@ struct vt_world {
 
 namespace vt {
 
+// Error reporting shared by the C-ABI translation units: record a detail
+// message for vt_last_error() and return the status.
+int fail(int status, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int hip_fail(hipError_t e, const char* what);
+bool aligned(const void* p, size_t a);
+void dev_free(void* p);
+
+#define VT_HIP(call)                                       \
+  do {                                                     \
+    hipError_t e_ = (call);                                \
+    if (e_ != hipSuccess) return ::vt::hip_fail(e_, #call); \
+  } while (0)
+
 // Per-launch shading constants.
 struct ShadeConsts {
   float sun[3];
@@ -46,7 +59,7 @@ struct ShadeConsts {
 };
 
 // Output pointers (any may be null).
-struct FrameOut {
+struct alignas(8) FrameOut {
   float4* shaded;
   uint32_t* hits;  // 3 words per pixel
   float4* color;
@@ -65,6 +78,14 @@ struct TileShape {
 
 DenseWorld dense_view(const vt_world* w);
 BrickmapWorld brickmap_view(const vt_world* w);
+
+// Host-side validation and translation of the C-ABI structs.
+int check_image_size(uint32_t W, uint32_t H, const char* who);
+int check_params(const vt_render_params& p, const vt_world* world, const char* who);
+int check_frame(const vt_frame* frame, const char* who);
+ViewConsts view_consts(const vt_camera* camera, uint32_t W, uint32_t H);
+ShadeConsts shade_consts(const vt_render_params& p);
+FrameOut frame_out(const vt_frame* frame);
 
 hipError_t launch_render_dense(const DenseWorld& w, uint32_t traversal, const ViewConsts& v,
                                const ShadeConsts& sc, const FrameOut& out, TileShape tile,

@@ -3,4 +3,6 @@ from . import _native, camera, scenes  # noqa: F401
 from ._native import (VT_FLAG_GBUF_UNORM8, VT_FLAG_LAMBERT, VT_TRAVERSAL_BITMASK,  # noqa: F401
                       VT_TRAVERSAL_FLAT, VtError, lib)
 from .camera import look_at  # noqa: F401
-from .render import Context, Frame, World, params, render  # noqa: F401
+from ._native import VT_TILE_8X8, VT_TILE_16X4, VT_TILE_32X2  # noqa: F401
+from .render import (Batch, Context, Frame, World, autotune, params, render,  # noqa: F401
+                     tuning)

@@ -60,9 +60,22 @@ class Frame(C.Structure):
                 ("illumination", C.c_void_p), ("steps", C.c_void_p), ("stats", C.c_void_p)]
 
 
+class View(C.Structure):
+    _fields_ = [("camera", Camera), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("_pad", C.c_uint32 * 2), ("frame", Frame)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("tile", C.c_uint32), ("chunk", C.c_uint32), ("traversal", C.c_uint32),
+                ("blocks_per_cu", C.c_uint32)]
+
+
+VT_TILE_8X8, VT_TILE_16X4, VT_TILE_32X2 = 3, 4, 5
+
 assert C.sizeof(Camera) == 96
 assert C.sizeof(RenderParams) == 48
 assert C.sizeof(Frame) == 64
+assert C.sizeof(View) == 176
 
 
 class VtError(RuntimeError):
@@ -97,6 +110,14 @@ def lib() -> C.CDLL:
         "vt_world_info": (i32, [vp, C.POINTER(u32), C.POINTER(C.c_size_t)]),
         "vt_render": (i32, [vp, vp, C.POINTER(Camera), C.POINTER(RenderParams),
                             C.POINTER(Frame), vp]),
+        "vt_batch_create": (i32, [vp, vp, C.POINTER(View), u32, C.POINTER(RenderParams),
+                                  C.POINTER(Tuning), vp, C.POINTER(vp)]),
+        "vt_batch_launch": (i32, [vp, vp]),
+        "vt_batch_capture_graph": (i32, [vp]),
+        "vt_batch_graph_launch": (i32, [vp, vp]),
+        "vt_batch_destroy": (i32, [vp]),
+        "vt_batch_autotune": (i32, [vp, vp, C.POINTER(View), u32, C.POINTER(RenderParams),
+                                    i32, C.POINTER(Tuning), C.POINTER(C.c_float)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -150,6 +150,65 @@ def params(width, height, max_primary=512, max_shadow=512, sun=SUN, flags=0, tra
     return p
 
 
+def tuning(tile=N.VT_TILE_8X8, chunk=1, traversal=N.VT_TRAVERSAL_FLAT, blocks_per_cu=8):
+    return N.Tuning(tile, chunk, traversal, blocks_per_cu)
+
+
+class Batch:
+    """Many views in one persistent launch (vt_batch_*); `views` is a list of
+    (camera, Frame) pairs whose frames stay alive as long as the batch."""
+
+    def __init__(self, ctx: Context, world: World, views, prm: N.RenderParams,
+                 tune: N.Tuning, stats=None):
+        self.ctx, self.world, self.frames = ctx, world, [f for _, f in views]
+        self.table = (N.View * len(views))()
+        for i, (cam, f) in enumerate(views):
+            self.table[i].camera = cam
+            self.table[i].width, self.table[i].height = f.width, f.height
+            self.table[i].frame = f.struct()
+        self.stats = stats
+        h = C.c_void_p()
+        N.check(N.lib().vt_batch_create(ctx.h, world.h, self.table, len(views), C.byref(prm),
+                                        C.byref(tune),
+                                        None if stats is None else stats.data_ptr(),
+                                        C.byref(h)), "vt_batch_create")
+        self.h = h
+
+    def launch(self, stream=None):
+        s = current_stream() if stream is None else stream
+        N.check(N.lib().vt_batch_launch(self.h, C.c_void_p(s)), "vt_batch_launch")
+
+    def capture_graph(self):
+        N.check(N.lib().vt_batch_capture_graph(self.h), "vt_batch_capture_graph")
+
+    def graph_launch(self, stream=None):
+        s = current_stream() if stream is None else stream
+        N.check(N.lib().vt_batch_graph_launch(self.h, C.c_void_p(s)), "vt_batch_graph_launch")
+
+    def close(self):
+        if self.h:
+            N.check(N.lib().vt_batch_destroy(self.h), "vt_batch_destroy")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(ctx: Context, world: World, views, prm: N.RenderParams, reps=3):
+    table = (N.View * len(views))()
+    for i, (cam, f) in enumerate(views):
+        table[i].camera = cam
+        table[i].width, table[i].height = f.width, f.height
+        table[i].frame = f.struct()
+    best, ms = N.Tuning(), C.c_float()
+    N.check(N.lib().vt_batch_autotune(ctx.h, world.h, table, len(views), C.byref(prm), reps,
+                                      C.byref(best), C.byref(ms)), "vt_batch_autotune")
+    return best, ms.value
+
+
 def current_stream():
     return _torch().cuda.current_stream().cuda_stream
 
