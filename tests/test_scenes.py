@@ -25,6 +25,11 @@ def test_world_bytes_pinned(name):
     assert sha(g) == m["sha256"]
 
 
+def test_generator_bytes_do_not_depend_on_threads():
+    s = S.spec("c0", n_sprites=200)
+    assert np.array_equal(S.generate(s, threads=1), S.generate(s, threads=5))
+
+
 def test_generator_is_deterministic_and_seeded():
     a = S.generate(S.spec("c0"))
     b = S.generate(S.spec("c0"))

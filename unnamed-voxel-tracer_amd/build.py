@@ -57,7 +57,7 @@ def build(verbose: bool = True) -> None:
     _run([hipcc, *HIP_FLAGS, "-o", os.path.join(LIB, "libvt.so"), *srcs], verbose)
     _run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-ffp-contract=off", "-Wall",
           "-o", os.path.join(LIB, "libvt_scenes.so"),
-          os.path.join(PKG, "scenes", "vt_scenes.c"), "-lm"], verbose)
+          os.path.join(PKG, "scenes", "vt_scenes.c"), "-lm", "-lpthread"], verbose)
 
 
 if __name__ == "__main__":

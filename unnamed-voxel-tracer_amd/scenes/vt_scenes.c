@@ -20,6 +20,8 @@
  * specified one so grids can be pinned by hash.
  */
 #include <math.h>
+#include <pthread.h>
+#include <stdatomic.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -116,21 +118,104 @@ static int cmp_double(const void* a, const void* b) {
   return (x > y) - (x < y);
 }
 
-/* Fills grid (dim^3 bytes, caller-allocated).  Returns 0 on success. */
-int vts_generate(const vts_spec* s, uint8_t* grid) {
+/* Sprite shapes are independent of each other, so they are cut in parallel
+ * (one solid-voxel bitmask per sprite); everything that consumes the random
+ * streams and the stamping itself stay sequential, so the bytes do not depend
+ * on the thread count. */
+typedef struct sprite_job {
+  const vts_spec* s;
+  const uint64_t* keys;
+  uint8_t* masks; /* n_sprites * size^3 / 8 */
+  size_t q;       /* rank of the threshold value */
+  atomic_uint next;
+} sprite_job;
+
+static void* sprite_worker(void* arg) {
+  sprite_job* J = (sprite_job*)arg;
+  const uint32_t S = J->s->sprite_size;
+  const size_t cnt = (size_t)S * S * S;
+  double* val = (double*)malloc(cnt * sizeof(double));
+  double* srt = (double*)malloc(cnt * sizeof(double));
+  if (!val || !srt) {
+    free(val);
+    free(srt);
+    return (void*)1;
+  }
+  for (;;) {
+    uint32_t i = atomic_fetch_add(&J->next, 1u);
+    if (i >= J->s->n_sprites) break;
+    uint64_t key = J->keys[i];
+    for (uint32_t z = 0; z < S; z++)
+      for (uint32_t y = 0; y < S; y++)
+        for (uint32_t x = 0; x < S; x++) {
+          double v = vnoise3((x + 0.5) * 0.25, (y + 0.5) * 0.25, (z + 0.5) * 0.25, key);
+          v += 0.5 * vnoise3((x + 0.5) * 0.5, (y + 0.5) * 0.5, (z + 0.5) * 0.5, key ^ 0x55ull);
+          val[x + S * (y + S * z)] = v;
+        }
+    memcpy(srt, val, cnt * sizeof(double));
+    qsort(srt, cnt, sizeof(double), cmp_double);
+    double thr = srt[J->q];
+    uint8_t* m = J->masks + (size_t)i * (cnt / 8);
+    memset(m, 0, cnt / 8);
+    for (size_t k = 0; k < cnt; k++)
+      if (val[k] > thr) m[k >> 3] |= (uint8_t)(1u << (k & 7));
+  }
+  free(val);
+  free(srt);
+  return NULL;
+}
+
+typedef struct terrain_job {
+  const vts_spec* s;
+  uint8_t* grid;
+  atomic_uint next;
+} terrain_job;
+
+static void* terrain_worker(void* arg) {
+  terrain_job* J = (terrain_job*)arg;
+  const uint64_t n = J->s->dim;
+  for (;;) {
+    uint32_t z = atomic_fetch_add(&J->next, 1u);
+    if (z >= n) break;
+    for (uint64_t x = 0; x < n; x++) {
+      uint32_t h = vts_height(J->s, (uint32_t)x, z);
+      for (uint64_t y = 0; y < h; y++) {
+        uint8_t m = (y + 1 == h) ? 1 : (y + 4 >= h ? 2 : 3);
+        J->grid[x + n * (y + n * (uint64_t)z)] = m;
+      }
+    }
+  }
+  return NULL;
+}
+
+static int run_threads(void* (*fn)(void*), void* arg, int threads) {
+  pthread_t th[64];
+  if (threads < 1) threads = 1;
+  if (threads > 64) threads = 64;
+  int started = 0, rc = 0;
+  for (; started < threads; started++)
+    if (pthread_create(&th[started], NULL, fn, arg) != 0) break;
+  if (started == 0) return fn(arg) != NULL;
+  for (int i = 0; i < started; i++) {
+    void* r = NULL;
+    pthread_join(th[i], &r);
+    if (r) rc = 1;
+  }
+  return rc;
+}
+
+/* Fills grid (dim^3 bytes, caller-allocated) using up to `threads` threads;
+ * the result does not depend on the thread count.  Returns 0 on success. */
+int vts_generate_mt(const vts_spec* s, uint8_t* grid, int threads) {
   const uint64_t n = s->dim;
   if (n == 0 || !grid) return 1;
   memset(grid, 0, n * n * n);
   /* terrain */
-  for (uint64_t z = 0; z < n; z++) {
-    for (uint64_t x = 0; x < n; x++) {
-      uint32_t h = vts_height(s, (uint32_t)x, (uint32_t)z);
-      for (uint64_t y = 0; y < h; y++) {
-        uint8_t m = (y + 1 == h) ? 1 : (y + 4 >= h ? 2 : 3);
-        grid[x + n * (y + n * z)] = m;
-      }
-    }
-  }
+  terrain_job tj;
+  tj.s = s;
+  tj.grid = grid;
+  atomic_init(&tj.next, 0u);
+  if (run_threads(terrain_worker, &tj, threads)) return 2;
   /* solid blocks */
   rng rb = rng_stream(s->seed, 1);
   for (uint32_t i = 0; i < s->n_blocks; i++) {
@@ -148,43 +233,54 @@ int vts_generate(const vts_spec* s, uint8_t* grid) {
   if (s->n_sprites) {
     const uint32_t S = s->sprite_size;
     const size_t cnt = (size_t)S * S * S;
-    double* val = (double*)malloc(cnt * sizeof(double));
-    double* srt = (double*)malloc(cnt * sizeof(double));
-    if (!val || !srt) {
-      free(val);
-      free(srt);
+    if (S == 0 || S > n || cnt % 8 != 0) return 3;
+    uint64_t* keys = (uint64_t*)malloc(sizeof(uint64_t) * s->n_sprites);
+    uint32_t* pos = (uint32_t*)malloc(sizeof(uint32_t) * 3 * s->n_sprites);
+    uint8_t* mat = (uint8_t*)malloc(s->n_sprites);
+    uint8_t* masks = (uint8_t*)malloc((cnt / 8) * s->n_sprites);
+    if (!keys || !pos || !mat || !masks) {
+      free(keys);
+      free(pos);
+      free(mat);
+      free(masks);
       return 2;
     }
     rng rs = rng_stream(s->seed, 2);
-    size_t q = (size_t)floor((1.0 - s->sprite_fill) * (double)cnt);
-    if (q >= cnt) q = cnt - 1;
     for (uint32_t i = 0; i < s->n_sprites; i++) {
-      uint64_t key = rng_next(&rs);
-      uint32_t sx = rng_range(&rs, 0, (uint32_t)n - S);
-      uint32_t sz = rng_range(&rs, 0, (uint32_t)n - S);
-      uint8_t m = (uint8_t)rng_range(&rs, 4, 8);
-      uint32_t sy = vts_height(s, sx + S / 2, sz + S / 2);
+      keys[i] = rng_next(&rs);
+      pos[3 * i + 0] = rng_range(&rs, 0, (uint32_t)n - S);
+      pos[3 * i + 2] = rng_range(&rs, 0, (uint32_t)n - S);
+      mat[i] = (uint8_t)rng_range(&rs, 4, 8);
+      pos[3 * i + 1] = vts_height(s, pos[3 * i] + S / 2, pos[3 * i + 2] + S / 2);
+    }
+    sprite_job sj;
+    sj.s = s;
+    sj.keys = keys;
+    sj.masks = masks;
+    sj.q = (size_t)floor((1.0 - s->sprite_fill) * (double)cnt);
+    if (sj.q >= cnt) sj.q = cnt - 1;
+    atomic_init(&sj.next, 0u);
+    int rc = run_threads(sprite_worker, &sj, threads);
+    for (uint32_t i = 0; rc == 0 && i < s->n_sprites; i++) {
+      const uint32_t sx = pos[3 * i], sy = pos[3 * i + 1], sz = pos[3 * i + 2];
+      const uint8_t* m = masks + (size_t)i * (cnt / 8);
       for (uint32_t z = 0; z < S; z++)
         for (uint32_t y = 0; y < S; y++)
           for (uint32_t x = 0; x < S; x++) {
-            double v = vnoise3((x + 0.5) * 0.25, (y + 0.5) * 0.25, (z + 0.5) * 0.25, key);
-            v += 0.5 * vnoise3((x + 0.5) * 0.5, (y + 0.5) * 0.5, (z + 0.5) * 0.5, key ^ 0x55ull);
-            val[x + S * (y + S * z)] = v;
-          }
-      memcpy(srt, val, cnt * sizeof(double));
-      qsort(srt, cnt, sizeof(double), cmp_double);
-      double thr = srt[q];
-      for (uint32_t z = 0; z < S; z++)
-        for (uint32_t y = 0; y < S; y++)
-          for (uint32_t x = 0; x < S; x++) {
+            const size_t k = x + S * (y + (size_t)S * z);
             uint64_t gy = (uint64_t)sy + y;
             if (gy >= n) continue;
-            if (val[x + S * (y + S * z)] > thr)
-              grid[(sx + x) + n * (gy + n * (uint64_t)(sz + z))] = m;
+            if ((m[k >> 3] >> (k & 7)) & 1u)
+              grid[(sx + x) + n * (gy + n * (uint64_t)(sz + z))] = mat[i];
           }
     }
-    free(val);
-    free(srt);
+    free(keys);
+    free(pos);
+    free(mat);
+    free(masks);
+    if (rc) return 2;
   }
   return 0;
 }
+
+int vts_generate(const vts_spec* s, uint8_t* grid) { return vts_generate_mt(s, grid, 1); }

@@ -54,6 +54,8 @@ def _lib():
         L = C.CDLL(path)
         L.vts_generate.restype = C.c_int
         L.vts_generate.argtypes = [C.POINTER(SceneSpec), C.c_void_p]
+        L.vts_generate_mt.restype = C.c_int
+        L.vts_generate_mt.argtypes = [C.POINTER(SceneSpec), C.c_void_p, C.c_int]
         L.vts_height.restype = C.c_uint32
         L.vts_height.argtypes = [C.POINTER(SceneSpec), C.c_uint32, C.c_uint32]
         _scn = L
@@ -70,10 +72,14 @@ def spec(name: str = None, **over) -> SceneSpec:
     return SceneSpec(**d)
 
 
-def generate(s: SceneSpec) -> np.ndarray:
+def generate(s: SceneSpec, threads: int = 0) -> np.ndarray:
+    """The grid of a spec; the bytes do not depend on `threads` (0 = the
+    process's CPU share, at most 16)."""
     n = int(s.dim)
     grid = np.empty((n, n, n), dtype=np.uint8)
-    rc = _lib().vts_generate(C.byref(s), grid.ctypes.data)
+    if threads <= 0:
+        threads = min(len(os.sched_getaffinity(0)), 16)
+    rc = _lib().vts_generate_mt(C.byref(s), grid.ctypes.data, int(threads))
     if rc != 0:
         raise RuntimeError(f"vts_generate failed ({rc})")
     return grid
