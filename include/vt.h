@@ -69,6 +69,13 @@ typedef struct vt_camera {
                                       /* as gbuffer.zig:10-13 declares them   */
 #define VT_FLAG_LAMBERT (1u << 1)     /* shaded = albedo*(ambient+max(n.l,0)*vis) */
                                       /* instead of the blit composite        */
+#define VT_FLAG_ALBEDO_COMPACT (1u << 2) /* albedo / its gradient hold occupied */
+                                      /* voxels only, in the order of the     */
+                                      /* world's compact index                */
+#define VT_FLAG_GRAD_L2 (1u << 3)     /* with LAMBERT: also accumulate the    */
+                                      /* gradients of sum |shaded - target|^2 */
+                                      /* (frame.target) into grad_albedo,     */
+                                      /* grad_sun and loss (recompute mode)   */
 
 /* traversal storage variants for dense worlds (identical hits by contract) */
 #define VT_TRAVERSAL_FLAT 0u    /* one uint8 gather per executed step        */
@@ -83,8 +90,12 @@ typedef struct vt_render_params {
   uint32_t flags;            /* VT_FLAG_*                                     */
   uint32_t traversal;        /* VT_TRAVERSAL_*                                */
   float ambient;             /* VT_FLAG_LAMBERT only                          */
-  const float* albedo;       /* VT_FLAG_LAMBERT only: device float[dim^3*3],  */
-                             /* RGB per voxel, index x + dim*(y + dim*z)      */
+  const float* albedo;       /* VT_FLAG_LAMBERT only: device float RGB per    */
+                             /* voxel: dense [dim^3*3], index x+dim*(y+dim*z), */
+                             /* or compact [occupied*3] (VT_FLAG_ALBEDO_COMPACT) */
+  float* grad_albedo;        /* VT_FLAG_GRAD_L2: += dL/dalbedo, albedo layout */
+  float* grad_sun;           /* VT_FLAG_GRAD_L2: += dL/dsun, device float[3]  */
+  float* loss;               /* VT_FLAG_GRAD_L2: += L, device float[1]        */
 } vt_render_params;
 
 /* ---- outputs ------------------------------------------------------------ */
@@ -133,6 +144,7 @@ typedef struct vt_frame {
   float* illumination;   /* W*H*4: G-buffer image 3 (frameIllumination)       */
   uint32_t* steps;       /* W*H: primary steps | shadow steps << 16           */
   unsigned long long* stats; /* VT_STAT_COUNT counters, accumulated (+=)      */
+  const float* target;   /* W*H*4 input: VT_FLAG_GRAD_L2 target image (RGB)   */
 } vt_frame;
 
 /* ---- context ------------------------------------------------------------ */
@@ -180,6 +192,35 @@ int vt_world_info(const vt_world* world, uint32_t* dim, size_t* device_bytes);
  * (blit.fragment.glsl:30-31), fused into one launch.                      */
 int vt_render(vt_ctx* ctx, const vt_world* world, const vt_camera* camera,
               const vt_render_params* params, const vt_frame* frame, void* stream);
+
+/* ---- differentiable Lambert shading ---------------------------------------- */
+/* Project-defined (no reference counterpart: the reference shades with the
+ * blit composite).  shaded = albedo[voxel] * (ambient + max(n . sun/|sun|, 0)
+ * * vis) on hit pixels, 0 on sky pixels; n = the hit face's normal, vis = 0
+ * when the shadow ray was occluded.  The discrete hit structure (voxel, face,
+ * shadow bit) is held fixed: gradients are those of the shading alone.      */
+
+/* Builds (once) the compact albedo index of a dense world: occupied voxels
+ * numbered in Morton order, through an exclusive prefix sum over the
+ * popcounts of the 4^3-brick occupancy words.  Returns the occupied count. */
+int vt_world_build_compact_index(vt_ctx* ctx, vt_world* world, uint64_t* n_occupied);
+
+/* upstream kinds for vt_lambert_backward */
+#define VT_UPSTREAM_GRADIENT 0u  /* upstream = dL/dshaded (RGBA32F, rgb used)   */
+#define VT_UPSTREAM_L2_TARGET 1u /* upstream = target T: L = sum |shaded-T|^2   */
+
+/* Shading from stored hit records (the forward of the backward below):
+ * out[i] = shaded of hits[i].  Uses params->albedo, sun, ambient, flags
+ * (VT_FLAG_ALBEDO_COMPACT).  npix records, out RGBA32F. */
+int vt_lambert_shade(vt_ctx* ctx, const vt_world* world, const vt_hit_record* hits,
+                     uint32_t npix, const vt_render_params* params, float* out, void* stream);
+/* Gradients from stored hit records: grad_albedo += dL/dalbedo (albedo
+ * layout), grad_sun[3] += dL/dsun, loss[0] += L (VT_UPSTREAM_L2_TARGET only;
+ * may be NULL).  Float atomics: sums are order-dependent to rounding. */
+int vt_lambert_backward(vt_ctx* ctx, const vt_world* world, const vt_hit_record* hits,
+                        uint32_t npix, const vt_render_params* params, const float* upstream,
+                        uint32_t upstream_kind, float* grad_albedo, float* grad_sun,
+                        float* loss, void* stream);
 
 /* ---- batched rendering --------------------------------------------------- */
 /* Many views of uneven resolution in ONE persistent launch: waves pull

@@ -17,11 +17,13 @@ ROOT = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "lib")
 CSRC = os.path.join(PKG, "csrc")
 
-HIP_SOURCES = ["vt_api.hip", "vt_render.hip", "vt_world.hip", "vt_batch.hip"]
+HIP_SOURCES = ["vt_api.hip", "vt_render.hip", "vt_world.hip", "vt_batch.hip", "vt_lambert.hip"]
 
 # -ffp-contract=off: every binary32 multiply and add rounds separately (the
 #   traversal must reproduce the CPU restatement bit for bit).
 # correctly rounded divide/sqrt: 1/d, normalize() and length() bit-exact.
+# -munsafe-fp-atomics: float atomicAdd as one global_atomic_add_f32 (device
+#   memory from hipMalloc is coarse-grained), not a CAS loop.
 # -packed-fp32-ops: no v_pk_*_f32 in the device code.
 HIP_FLAGS = [
     "--offload-arch=gfx950",
@@ -31,6 +33,7 @@ HIP_FLAGS = [
     "-shared",
     "-ffp-contract=off",
     "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-munsafe-fp-atomics",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
 ]

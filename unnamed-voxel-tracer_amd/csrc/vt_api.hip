@@ -45,6 +45,7 @@ using vt::hip_fail;
 
 void free_world(vt_world* w) {
   if (!w) return;
+  dev_free(w->prefix);
   dev_free(w->vox);
   dev_free(w->l0);
   dev_free(w->l1);
@@ -91,9 +92,11 @@ DenseWorld dense_view(const vt_world* w) {
   uint32_t bits = 0;
   while ((1u << bits) < w->dim) bits++;
   const uint32_t low = bits * 3 >= 32 ? 0xFFFFFFFFu : ((1u << (bits * 3)) - 1u);
+  d.log2dim = bits;
   d.mx = 0x09249249u & low;
   d.my = (0x09249249u << 1) & low;
   d.mz = (0x09249249u << 2) & low;
+  d.prefix = w->prefix;
   return d;
 }
 
@@ -120,8 +123,19 @@ int check_params(const vt_render_params& p, const vt_world* world, const char* w
   if (p.max_steps_primary < 0 || p.max_steps_primary > 65535 || p.max_steps_shadow < 0 ||
       p.max_steps_shadow > 65535)
     return fail(VT_ERR_INVALID_ARGUMENT, "%s: step caps must lie in [0, 65535]", who);
-  if (p.flags & ~(VT_FLAG_GBUF_UNORM8 | VT_FLAG_LAMBERT))
+  if (p.flags & ~(VT_FLAG_GBUF_UNORM8 | VT_FLAG_LAMBERT | VT_FLAG_ALBEDO_COMPACT |
+                  VT_FLAG_GRAD_L2))
     return fail(VT_ERR_INVALID_ARGUMENT, "%s: unknown flags 0x%x", who, p.flags);
+  if ((p.flags & (VT_FLAG_ALBEDO_COMPACT | VT_FLAG_GRAD_L2)) && !(p.flags & VT_FLAG_LAMBERT))
+    return fail(VT_ERR_INVALID_ARGUMENT, "%s: ALBEDO_COMPACT / GRAD_L2 need VT_FLAG_LAMBERT", who);
+  if ((p.flags & VT_FLAG_ALBEDO_COMPACT) && (world->kind != WorldKind::Dense || !world->prefix))
+    return fail(VT_ERR_UNSUPPORTED,
+                "%s: VT_FLAG_ALBEDO_COMPACT needs vt_world_build_compact_index first", who);
+  if ((p.flags & VT_FLAG_GRAD_L2) && !p.grad_albedo)
+    return fail(VT_ERR_NULL_POINTER, "%s: VT_FLAG_GRAD_L2 needs grad_albedo", who);
+  if ((p.grad_albedo && !aligned(p.grad_albedo, 4)) || (p.grad_sun && !aligned(p.grad_sun, 4)) ||
+      (p.loss && !aligned(p.loss, 4)))
+    return fail(VT_ERR_MISALIGNED, "%s: gradient outputs", who);
   if (p.traversal > VT_TRAVERSAL_BITMASK)
     return fail(VT_ERR_INVALID_ARGUMENT, "%s: unknown traversal %u", who, p.traversal);
   if (world->kind == WorldKind::Brickmap && p.traversal != VT_TRAVERSAL_FLAT)
@@ -136,7 +150,7 @@ int check_params(const vt_render_params& p, const vt_world* world, const char* w
 
 int check_frame(const vt_frame* frame, const char* who) {
   const void* f16[] = {frame->shaded, frame->color, frame->normal, frame->position,
-                       frame->illumination};
+                       frame->illumination, frame->target};
   for (const void* q : f16)
     if (q && !aligned(q, 16)) return fail(VT_ERR_MISALIGNED, "%s: image not 16B aligned", who);
   if ((frame->hits && !aligned(frame->hits, 4)) || (frame->steps && !aligned(frame->steps, 4)) ||
@@ -163,6 +177,9 @@ ShadeConsts shade_consts(const vt_render_params& p) {
   sc.flags = p.flags;
   sc.ambient = p.ambient;
   sc.albedo = p.albedo;
+  sc.grad_albedo = p.grad_albedo;
+  sc.grad_sun = p.grad_sun;
+  sc.loss = p.loss;
   return sc;
 }
 
@@ -176,6 +193,7 @@ FrameOut frame_out(const vt_frame* frame) {
   o.illum = reinterpret_cast<float4*>(frame->illumination);
   o.steps = frame->steps;
   o.stats = frame->stats;
+  o.target = reinterpret_cast<const float4*>(frame->target);
   return o;
 }
 
@@ -358,6 +376,8 @@ int vt_render(vt_ctx* ctx, const vt_world* world, const vt_camera* camera,
   int st = vt::check_image_size(p.width, p.height, "vt_render");
   if (st == VT_OK) st = vt::check_params(p, world, "vt_render");
   if (st == VT_OK) st = vt::check_frame(frame, "vt_render");
+  if (st == VT_OK && (p.flags & VT_FLAG_GRAD_L2) && !frame->target)
+    st = fail(VT_ERR_NULL_POINTER, "vt_render: VT_FLAG_GRAD_L2 needs frame->target");
   if (st != VT_OK) return st;
   VT_HIP(hipSetDevice(ctx->device));
   const vt::ViewConsts v = vt::view_consts(camera, p.width, p.height);

@@ -41,6 +41,7 @@ __global__ __launch_bounds__(256) void k_render_batch(DenseWorld w, ShadeConsts 
   const uint32_t tw = 1u << log2_tw, th = 64u >> log2_tw;
   const uint32_t home = blockIdx.x % kQueues;
   Counters cnt = {0, 0, 0, 0, 0, 0, 0, 0};
+  GradAcc gacc = {0.f, 0.f, 0.f, 0.f};
   for (uint32_t k = 0; k < kQueues; k++) {
     const uint32_t q = (home + k) % kQueues;
     const uint32_t lo = (uint32_t)(((uint64_t)total_tiles * q) / kQueues);
@@ -66,7 +67,8 @@ __global__ __launch_bounds__(256) void k_render_batch(DenseWorld w, ShadeConsts 
         const uint32_t px = tx * tw + (lane & (tw - 1u));
         const uint32_t py = ty * th + (lane >> log2_tw);
         const ViewConsts v = bv.v;
-        if (px < v.W && py < v.H) trace_pixel_dense<L, kStats>(w, v, sc, bv.out, px, py, cnt);
+        if (px < v.W && py < v.H)
+          trace_pixel_dense<L, kStats>(w, v, sc, bv.out, px, py, cnt, gacc);
       }
     }
   }
@@ -75,6 +77,7 @@ __global__ __launch_bounds__(256) void k_render_batch(DenseWorld w, ShadeConsts 
     o.stats = stats;
     flush_counters(o, cnt);
   }
+  if (sc.flags & VT_FLAG_GRAD_L2) flush_grad(sc, gacc);
 }
 
 }  // namespace vt
@@ -162,6 +165,8 @@ int vt_batch_create(vt_ctx* ctx, const vt_world* world, const vt_view* views, ui
     if ((st = vt::check_image_size(v.width, v.height, "vt_batch_create")) != VT_OK ||
         (st = vt::check_frame(&v.frame, "vt_batch_create")) != VT_OK)
       return st;
+    if ((p.flags & VT_FLAG_GRAD_L2) && !v.frame.target)
+      return fail(VT_ERR_NULL_POINTER, "vt_batch_create: view %u has no target", i);
     vt::BatchView& b = table[i];
     std::memset(&b, 0, sizeof b);
     b.v = vt::view_consts(&v.camera, v.width, v.height);

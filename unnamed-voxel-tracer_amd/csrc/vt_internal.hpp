@@ -32,6 +32,9 @@ struct vt_world {
   uint32_t* pool;
   uint32_t* atlas;
   size_t pool_words;
+  // compact albedo index of a dense world (built on request)
+  uint32_t* prefix;     // occupied voxels before each L0 word
+  uint64_t n_occupied;
 };
 
 namespace vt {
@@ -56,6 +59,9 @@ struct ShadeConsts {
   uint32_t flags;
   float ambient;
   const float* albedo;
+  float* grad_albedo;  // VT_FLAG_GRAD_L2 accumulators
+  float* grad_sun;
+  float* loss;
 };
 
 // Output pointers (any may be null).
@@ -68,6 +74,7 @@ struct alignas(8) FrameOut {
   float4* illum;
   uint32_t* steps;
   unsigned long long* stats;
+  const float4* target;  // VT_FLAG_GRAD_L2 input
 };
 
 // Wave tile: 64 lanes cover tile_w x (64 / tile_w) pixels; a 256-thread
@@ -97,5 +104,16 @@ hipError_t launch_render_brickmap(const BrickmapWorld& w, const ViewConsts& v,
 // World construction kernels.
 hipError_t launch_build_dense(const uint8_t* linear, uint32_t dim, uint8_t* vox, uint64_t* l0,
                               uint64_t* l1, hipStream_t stream);
+// Exclusive prefix sum of popcount(l0[i]) into prefix[0..nwords); scratch
+// holds ceil(nwords/1024) + 1 words; *total receives the sum (device).
+hipError_t launch_compact_prefix(const uint64_t* l0, uint64_t nwords, uint32_t* prefix,
+                                 uint32_t* scratch, hipStream_t stream);
+
+// Lambert shading / backward from stored hit records.
+hipError_t launch_lambert_shade(const DenseWorld& w, const ShadeConsts& sc, const uint32_t* hits,
+                                uint32_t npix, float4* out, hipStream_t stream);
+hipError_t launch_lambert_backward(const DenseWorld& w, const ShadeConsts& sc,
+                                   const uint32_t* hits, uint32_t npix, const float4* upstream,
+                                   uint32_t kind, hipStream_t stream);
 
 }  // namespace vt

@@ -20,6 +20,7 @@ struct Primary {
   uint32_t data;   // RGBA8 colour (packUnorm4x8 texel)
   uint32_t sub;
   uint32_t material;
+  uint32_t code;   // Morton code of the hit block (dense worlds)
   float hx, hy, hz;
   int steps;
 };
@@ -27,6 +28,95 @@ struct Primary {
 struct Counters {
   uint32_t psteps, ssteps, hits, occluded, l1f, l0f, leaf, ent;
 };
+
+// ---- Lambert x visibility shading and its gradients (project-defined) ----
+struct LambertSun {
+  float lx, ly, lz, ls;  // sun / |sun| and |sun|
+};
+
+__device__ __forceinline__ LambertSun lambert_sun(const float sun[3]) {
+  LambertSun s;
+  s.ls = len3(sun[0], sun[1], sun[2]);
+  s.lx = sun[0] / s.ls;
+  s.ly = sun[1] / s.ls;
+  s.lz = sun[2] / s.ls;
+  return s;
+}
+
+__device__ __forceinline__ float lambert_nd(int face, const LambertSun& s) {
+  float nx, ny, nz;
+  face_normal(face, nx, ny, nz);
+  float nd = nx * s.lx;
+  nd = nd + ny * s.ly;
+  nd = nd + nz * s.lz;
+  return nd;
+}
+
+__device__ __forceinline__ float lambert_factor(float ambient, float nd, bool occluded) {
+  return ambient + fmaxf(nd, 0.0f) * (occluded ? 0.0f : 1.0f);
+}
+
+// Compact albedo index of the voxel with Morton code `code`: occupied voxels
+// before its 4^3 brick plus those before it inside the brick.
+__device__ __forceinline__ uint32_t compact_index(const DenseWorld& w, uint32_t code) {
+  const uint32_t word = code >> 6, bit = code & 63u;
+  const uint64_t below = bit ? (w.l0[word] & (~0ull >> (64u - bit))) : 0ull;
+  return w.prefix[word] + (uint32_t)__popcll(below);
+}
+
+__device__ __forceinline__ uint32_t voxel_code(const DenseWorld& w, uint32_t voxel) {
+  const uint32_t m = w.dim - 1u;
+  return morton3(voxel & m, (voxel >> w.log2dim) & m, voxel >> (2u * w.log2dim));
+}
+
+__device__ __forceinline__ uint32_t albedo_index(const DenseWorld& w, uint32_t flags,
+                                                 uint32_t voxel, uint32_t code) {
+  return (flags & VT_FLAG_ALBEDO_COMPACT) ? compact_index(w, code) : voxel;
+}
+
+struct GradAcc {
+  float loss, sx, sy, sz;  // per-lane partial sums, flushed per wave
+};
+
+// Gradients of one hit pixel with dL/dI = g: dL/dA_c = g_c * fct, and, when
+// lit, dL/dsun = (sum_c g_c A_c) * (n - (n.l) l) / |sun|.
+__device__ __forceinline__ void lambert_accumulate(const ShadeConsts& sc, uint32_t aidx, int face,
+                                                   float nd, bool occluded, const float* A,
+                                                   const float g[3], float fct,
+                                                   const LambertSun& s, GradAcc& acc) {
+  float* ga = sc.grad_albedo + 3 * (size_t)aidx;
+  atomicAdd(ga + 0, g[0] * fct);
+  atomicAdd(ga + 1, g[1] * fct);
+  atomicAdd(ga + 2, g[2] * fct);
+  if (nd > 0.0f && !occluded) {
+    const float k = (g[0] * A[0] + g[1] * A[1] + g[2] * A[2]) / s.ls;
+    float nx, ny, nz;
+    face_normal(face, nx, ny, nz);
+    acc.sx += k * (nx - nd * s.lx);
+    acc.sy += k * (ny - nd * s.ly);
+    acc.sz += k * (nz - nd * s.lz);
+  }
+}
+
+__device__ __forceinline__ float wave_sum_f(float x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+
+// Every lane of the wave must call this.
+__device__ __forceinline__ void flush_grad(const ShadeConsts& sc, const GradAcc& acc) {
+  const float l = wave_sum_f(acc.loss), x = wave_sum_f(acc.sx), y = wave_sum_f(acc.sy),
+              z = wave_sum_f(acc.sz);
+  if ((threadIdx.x & 63u) == 0u) {
+    if (sc.loss) atomicAdd(sc.loss, l);
+    if (sc.grad_sun) {
+      atomicAdd(sc.grad_sun + 0, x);
+      atomicAdd(sc.grad_sun + 1, y);
+      atomicAdd(sc.grad_sun + 2, z);
+    }
+  }
+}
 
 // Shadow origin: position + normal * 0.001 with the values the G-buffer
 // round trip would give (secondary.comp.glsl:25,36-37).
@@ -45,11 +135,14 @@ __device__ __forceinline__ void shadow_origin(const Primary& p, uint32_t flags, 
   soz = pz + nz * 0.001f;
 }
 
+// `dw` is the dense world (Lambert shading reads its compact index), null for
+// the reference layout (which the host never runs with VT_FLAG_LAMBERT).
 __device__ __forceinline__ void write_pixel(const ViewConsts& v, const ShadeConsts& sc,
                                             const FrameOut& out, uint32_t px, uint32_t py,
                                             const Primary& p, bool shit, float shx, float shy,
                                             float shz, float sox, float soy, float soz,
-                                            int ssteps, Counters& cnt) {
+                                            int ssteps, Counters& cnt, const DenseWorld* dw,
+                                            GradAcc& gacc) {
   const size_t pix = (size_t)py * v.W + px;
   const bool unorm = (sc.flags & VT_FLAG_GBUF_UNORM8) != 0u;
   float4 col, nrm, pos, ill;
@@ -105,24 +198,31 @@ __device__ __forceinline__ void write_pixel(const ViewConsts& v, const ShadeCons
     out.hits[3 * pix + 1] = __float_as_uint(t);
     out.hits[3 * pix + 2] = info;
   }
-  if (out.shaded) {
-    float4 s;
-    if (sc.flags & VT_FLAG_LAMBERT) {
-      if (p.hit) {
-        float ls = len3(sc.sun[0], sc.sun[1], sc.sun[2]);
-        float lx = sc.sun[0] / ls, ly = sc.sun[1] / ls, lz = sc.sun[2] / ls;
-        float nx, ny, nz;
-        face_normal(p.face, nx, ny, nz);
-        float nd = nx * lx;
-        nd = nd + ny * ly;
-        nd = nd + nz * lz;
-        float fct = sc.ambient + fmaxf(nd, 0.0f) * (occl ? 0.0f : 1.0f);
-        const float* a = sc.albedo + 3 * (size_t)p.voxel;
-        s = make_float4(a[0] * fct, a[1] * fct, a[2] * fct, 1.0f);
-      } else {
-        s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (sc.flags & VT_FLAG_LAMBERT) {
+    float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const bool grad = (sc.flags & VT_FLAG_GRAD_L2) != 0u;
+    float4 T = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (grad) T = out.target[pix];
+    if (p.hit) {
+      const LambertSun ls = lambert_sun(sc.sun);
+      const float nd = lambert_nd(p.face, ls);
+      const float fct = lambert_factor(sc.ambient, nd, occl);
+      const uint32_t ai = albedo_index(*dw, sc.flags, p.voxel, p.code);
+      const float* a = sc.albedo + 3 * (size_t)ai;
+      s = make_float4(a[0] * fct, a[1] * fct, a[2] * fct, 1.0f);
+      if (grad) {
+        const float r[3] = {s.x - T.x, s.y - T.y, s.z - T.z};
+        gacc.loss += r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        const float g[3] = {2.0f * r[0], 2.0f * r[1], 2.0f * r[2]};
+        lambert_accumulate(sc, ai, p.face, nd, occl, a, g, fct, ls, gacc);
       }
-    } else if (p.hit) {
+    } else if (grad) {
+      gacc.loss += T.x * T.x + T.y * T.y + T.z * T.z;
+    }
+    if (out.shaded) out.shaded[pix] = s;
+  } else if (out.shaded) {
+    float4 s;
+    if (p.hit) {
       float r, g, b;
       sky(sc.sun, ill.x, ill.y, ill.z, r, g, b);
       s = make_float4(col.x + ill.w * r, col.y + ill.w * g, col.z + ill.w * b,
@@ -154,7 +254,8 @@ __device__ __forceinline__ void flush_counters(const FrameOut& out, const Counte
 template <class L, bool kStats>
 __device__ __forceinline__ void trace_pixel_dense(const DenseWorld& w, const ViewConsts& v,
                                                   const ShadeConsts& sc, const FrameOut& out,
-                                                  uint32_t px, uint32_t py, Counters& cnt) {
+                                                  uint32_t px, uint32_t py, Counters& cnt,
+                                                  GradAcc& gacc) {
   DenseRay r;
   L lk;
   DenseHit h;
@@ -184,6 +285,7 @@ __device__ __forceinline__ void trace_pixel_dense(const DenseWorld& w, const Vie
     p.material = lk.material(w, h.code, h.v, cnt.leaf);
     p.data = w.palette[p.material];
     p.sub = h.sub;
+    p.code = h.code;
     p.hx = h.hx;
     p.hy = h.hy;
     p.hz = h.hz;
@@ -200,7 +302,7 @@ __device__ __forceinline__ void trace_pixel_dense(const DenseWorld& w, const Vie
     }
     ssteps = h.steps;
   }
-  write_pixel(v, sc, out, px, py, p, shit, shx, shy, shz, sox, soy, soz, ssteps, cnt);
+  write_pixel(v, sc, out, px, py, p, shit, shx, shy, shz, sox, soy, soz, ssteps, cnt, &w, gacc);
 }
 
 }  // namespace vt

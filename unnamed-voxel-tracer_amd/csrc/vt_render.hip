@@ -27,8 +27,10 @@ __global__ __launch_bounds__(256) void k_render_dense(DenseWorld w, ViewConsts v
   const uint32_t px = bx * (2u * tw) + (wave & 1u) * tw + (lane & (tw - 1u));
   const uint32_t py = by * (2u * th) + (wave >> 1) * th + (lane >> log2_tw);
   Counters cnt = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (px < v.W && py < v.H) trace_pixel_dense<L, kStats>(w, v, sc, out, px, py, cnt);
+  GradAcc gacc = {0.f, 0.f, 0.f, 0.f};
+  if (px < v.W && py < v.H) trace_pixel_dense<L, kStats>(w, v, sc, out, px, py, cnt, gacc);
   if (kStats) flush_counters(out, cnt);
+  if (sc.flags & VT_FLAG_GRAD_L2) flush_grad(sc, gacc);
 }
 
 template <bool kStats>
@@ -83,7 +85,9 @@ __global__ __launch_bounds__(256) void k_render_brickmap(BrickmapWorld w, ViewCo
       }
       ssteps = s.steps;
     }
-    write_pixel(v, sc, out, px, py, p, shit, shx, shy, shz, sox, soy, soz, ssteps, cnt);
+    GradAcc gacc = {0.f, 0.f, 0.f, 0.f};
+    write_pixel(v, sc, out, px, py, p, shit, shx, shy, shz, sox, soy, soz, ssteps, cnt, nullptr,
+                gacc);
   }
   if (kStats) flush_counters(out, cnt);
 }

@@ -170,7 +170,10 @@ struct DenseWorld {
   const uint64_t* l1;       // 16^3 regions: bit per nonempty brick
   const uint32_t* palette;  // RGBA8 colour of material k (the solid atlas cube)
   uint32_t dim;             // voxels per axis (MAP_DIMENSION), a power of two
+  uint32_t log2dim;
   uint32_t mx, my, mz;      // dilated masks of x, y, z over 3*log2(dim) bits
+  const uint32_t* prefix;   // compact albedo index: occupied voxels before each
+                            // L0 word (null until built)
 };
 
 __device__ __forceinline__ uint32_t part1by2(uint32_t x) {

@@ -57,6 +57,99 @@ __global__ __launch_bounds__(256) void k_build_l1(const uint64_t* __restrict__ l
   l1[i] = mask;
 }
 
+// ---- compact albedo index: exclusive prefix sum of popcount(L0 word) ----
+// Three passes: per-tile sums (1024 words per 256-thread block), one block
+// scanning the tile sums, then each tile scanning its own words.
+constexpr uint32_t kScanTile = 1024;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_t* sh,
+                                                             uint32_t& total) {
+  const uint32_t t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (uint32_t off = 1; off < 256; off <<= 1) {
+    const uint32_t a = t >= off ? sh[t - off] : 0u;
+    __syncthreads();
+    sh[t] += a;
+    __syncthreads();
+  }
+  total = sh[255];
+  const uint32_t incl = sh[t];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(256) void k_popc_tile_sums(const uint64_t* __restrict__ l0,
+                                                        uint64_t nwords,
+                                                        uint32_t* __restrict__ sums) {
+  __shared__ uint32_t sh[256];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + threadIdx.x * 4u;
+  uint32_t s = 0;
+  for (uint32_t k = 0; k < 4; k++)
+    if (base + k < nwords) s += (uint32_t)__popcll(l0[base + k]);
+  uint32_t total;
+  (void)block_exclusive_scan_256(s, sh, total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tile_sums(uint32_t* __restrict__ sums,
+                                                         uint32_t ntiles,
+                                                         uint32_t* __restrict__ total) {
+  __shared__ uint32_t buf[1024];
+  const uint32_t t = threadIdx.x;
+  uint32_t carry = 0;
+  for (uint32_t c = 0; c < ntiles; c += 1024u) {
+    const uint32_t i = c + t;
+    const uint32_t v = i < ntiles ? sums[i] : 0u;
+    buf[t] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024u; off <<= 1) {
+      const uint32_t a = t >= off ? buf[t - off] : 0u;
+      __syncthreads();
+      buf[t] += a;
+      __syncthreads();
+    }
+    if (i < ntiles) sums[i] = carry + buf[t] - v;
+    carry += buf[1023];
+    __syncthreads();
+  }
+  if (t == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(256) void k_prefix_words(const uint64_t* __restrict__ l0,
+                                                      uint64_t nwords,
+                                                      const uint32_t* __restrict__ tile_off,
+                                                      uint32_t* __restrict__ prefix) {
+  __shared__ uint32_t sh[256];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + threadIdx.x * 4u;
+  uint32_t pc[4], s = 0;
+  for (uint32_t k = 0; k < 4; k++) {
+    pc[k] = base + k < nwords ? (uint32_t)__popcll(l0[base + k]) : 0u;
+    s += pc[k];
+  }
+  uint32_t total;
+  uint32_t run = tile_off[blockIdx.x] + block_exclusive_scan_256(s, sh, total);
+  for (uint32_t k = 0; k < 4; k++) {
+    if (base + k < nwords) prefix[base + k] = run;
+    run += pc[k];
+  }
+}
+
+hipError_t launch_compact_prefix(const uint64_t* l0, uint64_t nwords, uint32_t* prefix,
+                                 uint32_t* scratch, hipStream_t stream) {
+  const uint32_t ntiles = (uint32_t)((nwords + kScanTile - 1) / kScanTile);
+  hipLaunchKernelGGL(k_popc_tile_sums, dim3(ntiles), dim3(256), 0, stream, l0, nwords, scratch);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(1024), 0, stream, scratch, ntiles,
+                     scratch + ntiles);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_prefix_words, dim3(ntiles), dim3(256), 0, stream, l0, nwords, scratch,
+                     prefix);
+  return hipGetLastError();
+}
+
 hipError_t launch_build_dense(const uint8_t* linear, uint32_t dim, uint8_t* vox, uint64_t* l0,
                               uint64_t* l1, hipStream_t stream) {
   const uint64_t n = (uint64_t)dim * dim * dim;  // a power of two >= 4096

@@ -34,6 +34,10 @@ VT_ERR_WRONG_CONTEXT = 8
 
 VT_FLAG_GBUF_UNORM8 = 1 << 0
 VT_FLAG_LAMBERT = 1 << 1
+VT_FLAG_ALBEDO_COMPACT = 1 << 2
+VT_FLAG_GRAD_L2 = 1 << 3
+VT_UPSTREAM_GRADIENT = 0
+VT_UPSTREAM_L2_TARGET = 1
 VT_TRAVERSAL_FLAT = 0
 VT_TRAVERSAL_BITMASK = 1
 
@@ -51,13 +55,15 @@ class RenderParams(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32),
                 ("max_steps_primary", C.c_int32), ("max_steps_shadow", C.c_int32),
                 ("sun", C.c_float * 3), ("flags", C.c_uint32), ("traversal", C.c_uint32),
-                ("ambient", C.c_float), ("albedo", C.c_void_p)]
+                ("ambient", C.c_float), ("albedo", C.c_void_p), ("grad_albedo", C.c_void_p),
+                ("grad_sun", C.c_void_p), ("loss", C.c_void_p)]
 
 
 class Frame(C.Structure):
     _fields_ = [("shaded", C.c_void_p), ("hits", C.c_void_p), ("color", C.c_void_p),
                 ("normal", C.c_void_p), ("position", C.c_void_p),
-                ("illumination", C.c_void_p), ("steps", C.c_void_p), ("stats", C.c_void_p)]
+                ("illumination", C.c_void_p), ("steps", C.c_void_p), ("stats", C.c_void_p),
+                ("target", C.c_void_p)]
 
 
 class View(C.Structure):
@@ -73,9 +79,9 @@ class Tuning(C.Structure):
 VT_TILE_8X8, VT_TILE_16X4, VT_TILE_32X2 = 3, 4, 5
 
 assert C.sizeof(Camera) == 96
-assert C.sizeof(RenderParams) == 48
-assert C.sizeof(Frame) == 64
-assert C.sizeof(View) == 176
+assert C.sizeof(RenderParams) == 72
+assert C.sizeof(Frame) == 72
+assert C.sizeof(View) == 184
 
 
 class VtError(RuntimeError):
@@ -110,6 +116,10 @@ def lib() -> C.CDLL:
         "vt_world_info": (i32, [vp, C.POINTER(u32), C.POINTER(C.c_size_t)]),
         "vt_render": (i32, [vp, vp, C.POINTER(Camera), C.POINTER(RenderParams),
                             C.POINTER(Frame), vp]),
+        "vt_world_build_compact_index": (i32, [vp, vp, C.POINTER(C.c_uint64)]),
+        "vt_lambert_shade": (i32, [vp, vp, vp, u32, C.POINTER(RenderParams), vp, vp]),
+        "vt_lambert_backward": (i32, [vp, vp, vp, u32, C.POINTER(RenderParams), vp, u32, vp, vp,
+                                      vp, vp]),
         "vt_batch_create": (i32, [vp, vp, C.POINTER(View), u32, C.POINTER(RenderParams),
                                   C.POINTER(Tuning), vp, C.POINTER(vp)]),
         "vt_batch_launch": (i32, [vp, vp]),

@@ -109,6 +109,7 @@ class Frame:
     illumination: object = None
     steps: object = None
     stats: object = None
+    target: object = None
 
     @classmethod
     def alloc(cls, width, height, device=0, shaded=True, hits=True, gbuffer=False, steps=False,
@@ -132,7 +133,8 @@ class Frame:
     def struct(self) -> N.Frame:
         p = lambda t: None if t is None else t.data_ptr()
         return N.Frame(p(self.shaded), p(self.hits), p(self.color), p(self.normal),
-                       p(self.position), p(self.illumination), p(self.steps), p(self.stats))
+                       p(self.position), p(self.illumination), p(self.steps), p(self.stats),
+                       p(self.target))
 
     def stats_dict(self):
         s = self.stats.cpu().numpy()
@@ -140,14 +142,43 @@ class Frame:
 
 
 def params(width, height, max_primary=512, max_shadow=512, sun=SUN, flags=0, traversal=0,
-           ambient=0.0, albedo=None) -> N.RenderParams:
+           ambient=0.0, albedo=None, grad_albedo=None, grad_sun=None, loss=None
+           ) -> N.RenderParams:
     p = N.RenderParams()
     p.width, p.height = width, height
     p.max_steps_primary, p.max_steps_shadow = max_primary, max_shadow
     p.sun[:] = [float(np.float32(x)) for x in sun]
     p.flags, p.traversal, p.ambient = flags, traversal, float(ambient)
-    p.albedo = None if albedo is None else albedo.data_ptr()
+    ptr = lambda t: None if t is None else t.data_ptr()
+    p.albedo, p.grad_albedo, p.grad_sun, p.loss = (ptr(albedo), ptr(grad_albedo), ptr(grad_sun),
+                                                   ptr(loss))
     return p
+
+
+def build_compact_index(ctx: Context, world: World) -> int:
+    n = C.c_uint64()
+    N.check(N.lib().vt_world_build_compact_index(ctx.h, world.h, C.byref(n)),
+            "vt_world_build_compact_index")
+    return n.value
+
+
+def lambert_shade(ctx: Context, world: World, hits, prm: N.RenderParams, out, stream=None):
+    """out[i] = Lambert shading of hit record i (hits: int32 [..., 3] tensor)."""
+    s = current_stream() if stream is None else stream
+    N.check(N.lib().vt_lambert_shade(ctx.h, world.h, hits.data_ptr(), hits.numel() // 3,
+                                     C.byref(prm), out.data_ptr(), C.c_void_p(s)),
+            "vt_lambert_shade")
+    return out
+
+
+def lambert_backward(ctx: Context, world: World, hits, prm: N.RenderParams, upstream, kind,
+                     grad_albedo, grad_sun=None, loss=None, stream=None):
+    s = current_stream() if stream is None else stream
+    ptr = lambda t: None if t is None else t.data_ptr()
+    N.check(N.lib().vt_lambert_backward(ctx.h, world.h, hits.data_ptr(), hits.numel() // 3,
+                                        C.byref(prm), upstream.data_ptr(), kind,
+                                        grad_albedo.data_ptr(), ptr(grad_sun), ptr(loss),
+                                        C.c_void_p(s)), "vt_lambert_backward")
 
 
 def tuning(tile=N.VT_TILE_8X8, chunk=1, traversal=N.VT_TRAVERSAL_FLAT, blocks_per_cu=8):
