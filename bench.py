@@ -156,15 +156,280 @@ def bench_batched(args):
     ctx.close()
 
 
+def _part1by2(t):
+    t = t & 0x3FF
+    t = (t | (t << 16)) & 0xFF0000FF
+    t = (t | (t << 8)) & 0x0300F00F
+    t = (t | (t << 4)) & 0x030C30C3
+    return (t | (t << 2)) & 0x09249249
+
+
+def _ground_truth_albedo(torch, grid, pal, compact, seed):
+    """Material base colour + N(0, 0.05) per voxel, float32 RGB.
+
+    dense:   dim^3 x 3, indexed by voxel id x + dim*(y + dim*z);
+    compact: n_occupied x 3, occupied voxels in Morton order (the order of the
+             library's occupancy prefix), built on the GPU.
+    """
+    base = torch.tensor([[(int(p) >> s) & 0xFF for s in (0, 8, 16)] for p in pal],
+                        dtype=torch.float32, device="cuda") / 255.0
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    g = torch.from_numpy(grid.reshape(-1)).cuda()
+    if compact:
+        dim = grid.shape[0]
+        lg = dim.bit_length() - 1
+        occ = torch.nonzero(g).squeeze(1)
+        code = (_part1by2(occ & (dim - 1)) | (_part1by2((occ >> lg) & (dim - 1)) << 1) |
+                (_part1by2(occ >> (2 * lg)) << 2))
+        order = torch.argsort(code)
+        mats = g[occ[order]].long()
+        del occ, code, order
+    else:
+        mats = g.long()
+    del g
+    a = base[mats]
+    a += 0.05 * torch.randn(a.shape, generator=gen, device="cuda")
+    return a.reshape(-1).contiguous()
+
+
+def bench_diff(args, scene):
+    """Inverse-rendering step: Lambert x shadow forward over every view, then
+    dL/dalbedo and dL/dsun of L = sum ||I - I_gt||^2 with the hit structure
+    held fixed.  Albedo starts at 0.5; targets are renders of the ground truth
+    (material colour + N(0, 0.05)).
+
+    c3: 256^3 world, 16 orbit views of 512x512, dense per-voxel albedo, one
+        vt_render per view;
+    c4: 1024^3 world, 48 uneven views (sides in [256,1024]), albedo only for
+        occupied voxels (compact prefix index), one persistent batch launch
+        per pass (captured in a hipGraph).
+
+    Timed modes (ms per view): plain forward (primary + shadow, no shading
+    model); stored = forward writing hit records + one backward launch over
+    all records; recompute = forward with the fused L2 gradient (no records).
+    """
+    import torch
+
+    import vt
+    from vt import scenes as S
+
+    batched = scene == "c4"
+    print(f"[bench] generating {scene} world", file=sys.stderr, flush=True)
+    grid = S.generate(S.spec(scene))
+    pal = S.palette()
+    ctx = vt.Context(0)
+    world = vt.World.dense(ctx, grid, pal)
+    views = S.views(scene)
+    cap = S.MAX_STEPS[scene]
+    nviews = len(views)
+    npix = sum(W * H for _, W, H in views)
+    stream = torch.cuda.current_stream()
+    flags = vt.VT_FLAG_LAMBERT
+    n_occ = None
+    if batched:
+        n_occ = vt.build_compact_index(ctx, world)
+        flags |= vt.VT_FLAG_ALBEDO_COMPACT
+    gt = _ground_truth_albedo(torch, grid, pal, batched, 0x5EED)
+    if batched:
+        assert gt.numel() == 3 * n_occ, (gt.numel(), n_occ)
+    del grid
+    albedo = torch.full_like(gt, 0.5)
+    grad = torch.zeros_like(gt)
+    gsun = torch.zeros(3, dtype=torch.float32, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+
+    # one buffer per image kind, views are slices: one backward launch covers all
+    hits = torch.empty((npix, 3), dtype=torch.int32, device="cuda")
+    shaded = torch.empty((npix, 4), dtype=torch.float32, device="cuda")
+    target = torch.empty((npix, 4), dtype=torch.float32, device="cuda")
+    frames, off = [], 0
+    for _, W, H in views:
+        f = vt.Frame(W, H)
+        f.hits = hits[off:off + W * H].view(H, W, 3)
+        f.shaded = shaded[off:off + W * H].view(H, W, 4)
+        f.target = target[off:off + W * H].view(H, W, 4)
+        frames.append(f)
+        off += W * H
+    cams = [c for c, _, _ in views]
+
+    def prm(W, H, fl, alb, with_grad=False):
+        g = (grad, gsun, loss) if with_grad else (None, None, None)
+        return vt.params(W, H, cap, cap, S.SUN, fl, 0, AMBIENT_DIFF, alb, *g)
+
+    def frames_without(*drop):
+        out = []
+        for f in frames:
+            c = vt.Frame(f.width, f.height, shaded=f.shaded, hits=f.hits, target=f.target)
+            for k in drop:
+                setattr(c, k, None)
+            out.append(c)
+        return out
+
+    # targets: renders of the ground truth
+    print("[bench] rendering targets", file=sys.stderr, flush=True)
+    tgt_frames = [vt.Frame(f.width, f.height, shaded=f.target) for f in frames]
+    for cam, f in zip(cams, tgt_frames):
+        vt.render(ctx, world, cam, f, prm(f.width, f.height, flags, gt))
+    torch.cuda.synchronize()
+    del gt
+
+    stats = torch.zeros(8, dtype=torch.int64, device="cuda")
+    for cam, f in zip(cams, frames):
+        fs = vt.Frame(f.width, f.height, hits=f.hits, stats=stats)
+        vt.render(ctx, world, cam, fs, prm(f.width, f.height, 0, None))
+    torch.cuda.synchronize()
+    st = dict(zip(vt._native.STAT_NAMES, stats.cpu().tolist()))
+    nhit = st["primary_hits"]
+
+    def zero():
+        grad.zero_()
+        gsun.zero_()
+        loss.zero_()
+
+    if batched:
+        plain_b = vt.Batch(ctx, world, list(zip(cams, frames_without("target"))),
+                           prm(0, 0, 0, None), vt.tuning())
+        fwd_b = vt.Batch(ctx, world, list(zip(cams, frames_without("target"))),
+                         prm(0, 0, flags, albedo), vt.tuning())
+        rec_b = vt.Batch(ctx, world, list(zip(cams, frames_without("hits"))),
+                         prm(0, 0, flags | vt.VT_FLAG_GRAD_L2, albedo, True), vt.tuning())
+        for b in (plain_b, fwd_b, rec_b):
+            b.capture_graph()
+        plain_fwd = plain_b.graph_launch
+        lambert_fwd = fwd_b.graph_launch
+        fused = rec_b.graph_launch
+    else:
+        pp = [prm(f.width, f.height, 0, None) for f in frames]
+        lp = [prm(f.width, f.height, flags, albedo) for f in frames]
+        gp = [prm(f.width, f.height, flags | vt.VT_FLAG_GRAD_L2, albedo, True) for f in frames]
+        f_plain, f_rec = frames_without("target"), frames_without("hits")
+
+        def plain_fwd():
+            for cam, f, p in zip(cams, f_plain, pp):
+                vt.render(ctx, world, cam, f, p)
+
+        def lambert_fwd():
+            for cam, f, p in zip(cams, f_plain, lp):
+                vt.render(ctx, world, cam, f, p)
+
+        def fused():
+            for cam, f, p in zip(cams, f_rec, gp):
+                vt.render(ctx, world, cam, f, p)
+
+    bprm = prm(0, 0, flags, albedo)
+
+    def backward():
+        vt.lambert_backward(ctx, world, hits, bprm, target, vt.VT_UPSTREAM_L2_TARGET, grad, gsun,
+                            loss, stream=stream.cuda_stream)
+
+    def stored():
+        zero()
+        lambert_fwd()
+        backward()
+
+    def recompute():
+        zero()
+        fused()
+
+    steps, warm = args.steps, args.warmup
+    print("[bench] plain forward", file=sys.stderr, flush=True)
+    plain_ms = timed(torch, stream, plain_fwd, steps, warm)
+    print("[bench] stored-record fwd+bwd", file=sys.stderr, flush=True)
+    stored_ms = timed(torch, stream, stored, steps, warm)
+    g_stored, s_stored, l_stored = grad.clone(), gsun.clone(), float(loss.item())
+    lambert_ms = timed(torch, stream, lambert_fwd, steps, warm)
+    lambert_fwd()
+    bwd_ms = timed(torch, stream, backward, steps, warm)
+    print("[bench] recompute fwd+bwd", file=sys.stderr, flush=True)
+    rec_ms = timed(torch, stream, recompute, steps, warm)
+    l_rec = float(loss.item())
+    agree = bool(torch.allclose(grad, g_stored, rtol=1e-4, atol=1e-4) and
+                 abs(l_rec - l_stored) <= 1e-4 * abs(l_stored))
+    print(f"[bench] loss stored {l_stored:.6g} recompute {l_rec:.6g} agree={agree} "
+          f"gsun {s_stored.tolist()}", file=sys.stderr, flush=True)
+
+    # a few untimed descent steps: the gradients lower the loss
+    losses = []
+    for _ in range(6):
+        stored()
+        torch.cuda.synchronize()
+        losses.append(float(loss.item()))
+        albedo.sub_(0.02 * grad)
+    torch.cuda.synchronize()
+
+    world_b = world.device_bytes()
+    alb_b = albedo.numel() * 4
+    img_b = npix * 16
+    resident = {
+        "stored_bytes": world_b + 2 * alb_b + 2 * img_b + npix * 12,
+        "recompute_bytes": world_b + 2 * alb_b + 2 * img_b,
+    }
+    rays = npix + nhit
+    # backward: 12 B record + 16 B target per pixel, 12 B albedo + 12 B gradient per hit
+    bwd_bytes = 28 * npix + 24 * nhit
+    fwd_bytes = (st["primary_steps"] + st["shadow_steps"] + OUT_BYTES_PER_PIXEL * npix +
+                 12 * nhit)
+    fwd_gbs = fwd_bytes / (lambert_ms * 1e-3) / 1e9
+    line = {
+        "metric": f"fwd+bwd ms per view ({scene}: {'1024' if batched else '256'}^3, "
+                  f"{nviews} views) vs plain forward",
+        "value": round(stored_ms / nviews, 4),
+        "unit": "ms/view",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warm,
+        "ms_per_step": round(stored_ms, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{scene}: {'1024^3 terrain x8 + 32000 sprites' if batched else '256^3 terrain x2 + 2400 blocks'}, "
+                         f"{nviews} views, {npix} pixels, Lambert x shadow, L2 to ground-truth "
+                         f"renders, albedo {'compact (' + str(n_occ) + ' occupied voxels)' if batched else 'dense'}"),
+            "launch": "persistent batch, hipGraph" if batched else "one vt_render per view",
+            "plain_fwd_ms_per_view": round(plain_ms / nviews, 4),
+            "lambert_fwd_ms_per_view": round(lambert_ms / nviews, 4),
+            "backward_ms_per_view": round(bwd_ms / nviews, 4),
+            "stored_fwd_bwd_ms_per_view": round(stored_ms / nviews, 4),
+            "recompute_fwd_bwd_ms_per_view": round(rec_ms / nviews, 4),
+            "fwd_bwd_over_plain": round(stored_ms / plain_ms, 3),
+            "recompute_over_plain": round(rec_ms / plain_ms, 3),
+            "modes_agree": agree,
+            "loss_descent": [round(x, 2) for x in losses],
+            "resident_hbm_bytes": resident,
+            "torch_peak_bytes": int(torch.cuda.max_memory_allocated()),
+            "backward_gbs": round(bwd_bytes / (bwd_ms * 1e-3) / 1e9, 2),
+            "rays": rays,
+        },
+        "roofline": {"bound": "hbm", "achieved": round(fwd_gbs, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(fwd_gbs / HBM_PEAK_GBS, 5), "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    if batched:
+        for b in (plain_b, fwd_b, rec_b):
+            b.close()
+    world.close()
+    ctx.close()
+
+
+AMBIENT_DIFF = 0.2
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--traversal", choices=["flat", "bitmask"], default="flat")
-    ap.add_argument("--workload", choices=["frame", "batched"], default="frame",
+    ap.add_argument("--workload", choices=["frame", "batched", "diff", "diff_batched"],
+                    default="frame",
                     help="frame: 512^3, one 1080p view (the headline); batched: 1024^3, 64 "
-                         "uneven cameras in one persistent graph launch")
+                         "uneven cameras in one persistent graph launch; diff: 256^3, 16 "
+                         "views fwd+bwd; diff_batched: 1024^3 compact albedo, 48 views fwd+bwd")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -174,6 +439,10 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "diff":
+        return bench_diff(args, "c3")
+    if args.workload == "diff_batched":
+        return bench_diff(args, "c4")
 
     import numpy as np
     import torch

@@ -30,7 +30,7 @@ __global__ void k_batch_reset(uint32_t* __restrict__ queues) {
   if (threadIdx.x < kQueues) queues[threadIdx.x] = 0u;
 }
 
-template <class L, bool kStats>
+template <class L, bool kStats, int kShade>
 __global__ __launch_bounds__(256) void k_render_batch(DenseWorld w, ShadeConsts sc,
                                                       const BatchView* __restrict__ views,
                                                       uint32_t n_views, uint32_t total_tiles,
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(256) void k_render_batch(DenseWorld w, ShadeConsts 
         const uint32_t py = ty * th + (lane >> log2_tw);
         const ViewConsts v = bv.v;
         if (px < v.W && py < v.H)
-          trace_pixel_dense<L, kStats>(w, v, sc, bv.out, px, py, cnt, gacc);
+          trace_pixel_dense<L, kStats, kShade>(w, v, sc, bv.out, px, py, cnt, gacc);
       }
     }
   }
@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256) void k_render_batch(DenseWorld w, ShadeConsts 
     o.stats = stats;
     flush_counters(o, cnt);
   }
-  if (sc.flags & VT_FLAG_GRAD_L2) flush_grad(sc, gacc);
+  if constexpr (kShade == kShadeLambertGrad) flush_grad(sc, gacc);
 }
 
 }  // namespace vt
@@ -108,16 +108,23 @@ hipError_t enqueue(const vt_batch* b, hipStream_t s) {
   if (e != hipSuccess) return e;
   const dim3 grid(b->nblocks), block(256);
   const uint32_t tile = b->tuning.tile, chunk = b->tuning.chunk;
-#define VT_BATCH_LAUNCH(L, S)                                                                  \
-  hipLaunchKernelGGL((vt::k_render_batch<vt::L, S>), grid, block, 0, s, w, b->sc, b->d_views, \
-                     b->n_views, b->total_tiles, b->d_queues, chunk, tile, b->stats)
-  if (b->tuning.traversal == VT_TRAVERSAL_BITMASK) {
-    if (b->stats) VT_BATCH_LAUNCH(BitmaskLookup, true);
-    else VT_BATCH_LAUNCH(BitmaskLookup, false);
-  } else {
-    if (b->stats) VT_BATCH_LAUNCH(FlatLookup, true);
-    else VT_BATCH_LAUNCH(FlatLookup, false);
+#define VT_BATCH_LAUNCH(L, S, M)                                                         \
+  hipLaunchKernelGGL((vt::k_render_batch<vt::L, S, vt::M>), grid, block, 0, s, w, b->sc,     \
+                     b->d_views, b->n_views, b->total_tiles, b->d_queues, chunk, tile, b->stats)
+#define VT_BATCH_SHADE(L, S)                                                            \
+  switch (vt::shade_mode(b->sc.flags)) {                                                \
+    case vt::kShadeLambertGrad: VT_BATCH_LAUNCH(L, S, kShadeLambertGrad); break;        \
+    case vt::kShadeLambert: VT_BATCH_LAUNCH(L, S, kShadeLambert); break;                \
+    default: VT_BATCH_LAUNCH(L, S, kShadeRef);                                          \
   }
+  if (b->tuning.traversal == VT_TRAVERSAL_BITMASK) {
+    if (b->stats) VT_BATCH_SHADE(BitmaskLookup, true)
+    else VT_BATCH_SHADE(BitmaskLookup, false)
+  } else {
+    if (b->stats) VT_BATCH_SHADE(FlatLookup, true)
+    else VT_BATCH_SHADE(FlatLookup, false)
+  }
+#undef VT_BATCH_SHADE
 #undef VT_BATCH_LAUNCH
   return hipGetLastError();
 }

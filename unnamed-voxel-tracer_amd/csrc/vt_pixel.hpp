@@ -137,6 +137,16 @@ __device__ __forceinline__ void shadow_origin(const Primary& p, uint32_t flags, 
 
 // `dw` is the dense world (Lambert shading reads its compact index), null for
 // the reference layout (which the host never runs with VT_FLAG_LAMBERT).
+// Shading model of a launch, fixed at compile time so the reference composite
+// carries no Lambert or gradient code (and no registers for it).
+enum ShadeMode : int { kShadeRef = 0, kShadeLambert = 1, kShadeLambertGrad = 2 };
+
+__host__ __device__ constexpr int shade_mode(uint32_t flags) {
+  return (flags & VT_FLAG_GRAD_L2) ? kShadeLambertGrad
+                                   : ((flags & VT_FLAG_LAMBERT) ? kShadeLambert : kShadeRef);
+}
+
+template <int kShade>
 __device__ __forceinline__ void write_pixel(const ViewConsts& v, const ShadeConsts& sc,
                                             const FrameOut& out, uint32_t px, uint32_t py,
                                             const Primary& p, bool shit, float shx, float shy,
@@ -198,11 +208,11 @@ __device__ __forceinline__ void write_pixel(const ViewConsts& v, const ShadeCons
     out.hits[3 * pix + 1] = __float_as_uint(t);
     out.hits[3 * pix + 2] = info;
   }
-  if (sc.flags & VT_FLAG_LAMBERT) {
+  if constexpr (kShade != kShadeRef) {
     float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const bool grad = (sc.flags & VT_FLAG_GRAD_L2) != 0u;
+    constexpr bool grad = kShade == kShadeLambertGrad;
     float4 T = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (grad) T = out.target[pix];
+    if constexpr (grad) T = out.target[pix];
     if (p.hit) {
       const LambertSun ls = lambert_sun(sc.sun);
       const float nd = lambert_nd(p.face, ls);
@@ -210,13 +220,13 @@ __device__ __forceinline__ void write_pixel(const ViewConsts& v, const ShadeCons
       const uint32_t ai = albedo_index(*dw, sc.flags, p.voxel, p.code);
       const float* a = sc.albedo + 3 * (size_t)ai;
       s = make_float4(a[0] * fct, a[1] * fct, a[2] * fct, 1.0f);
-      if (grad) {
+      if constexpr (grad) {
         const float r[3] = {s.x - T.x, s.y - T.y, s.z - T.z};
         gacc.loss += r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
         const float g[3] = {2.0f * r[0], 2.0f * r[1], 2.0f * r[2]};
         lambert_accumulate(sc, ai, p.face, nd, occl, a, g, fct, ls, gacc);
       }
-    } else if (grad) {
+    } else if constexpr (grad) {
       gacc.loss += T.x * T.x + T.y * T.y + T.z * T.z;
     }
     if (out.shaded) out.shaded[pix] = s;
@@ -251,7 +261,7 @@ __device__ __forceinline__ void flush_counters(const FrameOut& out, const Counte
 
 // One pixel of a dense world: primary ray (primary.comp.glsl:31-68), then the
 // shadow ray of a hit (secondary.comp.glsl:25-50), then every store.
-template <class L, bool kStats>
+template <class L, bool kStats, int kShade>
 __device__ __forceinline__ void trace_pixel_dense(const DenseWorld& w, const ViewConsts& v,
                                                   const ShadeConsts& sc, const FrameOut& out,
                                                   uint32_t px, uint32_t py, Counters& cnt,
@@ -302,7 +312,8 @@ __device__ __forceinline__ void trace_pixel_dense(const DenseWorld& w, const Vie
     }
     ssteps = h.steps;
   }
-  write_pixel(v, sc, out, px, py, p, shit, shx, shy, shz, sox, soy, soz, ssteps, cnt, &w, gacc);
+  write_pixel<kShade>(v, sc, out, px, py, p, shit, shx, shy, shz, sox, soy, soz, ssteps, cnt, &w,
+                     gacc);
 }
 
 }  // namespace vt

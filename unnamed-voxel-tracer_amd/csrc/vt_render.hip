@@ -16,7 +16,7 @@
 
 namespace vt {
 
-template <class L, bool kStats>
+template <class L, bool kStats, int kShade>
 __global__ __launch_bounds__(256) void k_render_dense(DenseWorld w, ViewConsts v, ShadeConsts sc,
                                                       FrameOut out, uint32_t log2_tw,
                                                       uint32_t blocks_x) {
@@ -28,9 +28,10 @@ __global__ __launch_bounds__(256) void k_render_dense(DenseWorld w, ViewConsts v
   const uint32_t py = by * (2u * th) + (wave >> 1) * th + (lane >> log2_tw);
   Counters cnt = {0, 0, 0, 0, 0, 0, 0, 0};
   GradAcc gacc = {0.f, 0.f, 0.f, 0.f};
-  if (px < v.W && py < v.H) trace_pixel_dense<L, kStats>(w, v, sc, out, px, py, cnt, gacc);
+  if (px < v.W && py < v.H)
+    trace_pixel_dense<L, kStats, kShade>(w, v, sc, out, px, py, cnt, gacc);
   if (kStats) flush_counters(out, cnt);
-  if (sc.flags & VT_FLAG_GRAD_L2) flush_grad(sc, gacc);
+  if constexpr (kShade == kShadeLambertGrad) flush_grad(sc, gacc);
 }
 
 template <bool kStats>
@@ -86,8 +87,8 @@ __global__ __launch_bounds__(256) void k_render_brickmap(BrickmapWorld w, ViewCo
       ssteps = s.steps;
     }
     GradAcc gacc = {0.f, 0.f, 0.f, 0.f};
-    write_pixel(v, sc, out, px, py, p, shit, shx, shy, shz, sox, soy, soz, ssteps, cnt, nullptr,
-                gacc);
+    write_pixel<kShadeRef>(v, sc, out, px, py, p, shit, shx, shy, shz, sox, soy, soz, ssteps, cnt,
+                           nullptr, gacc);
   }
   if (kStats) flush_counters(out, cnt);
 }
@@ -99,27 +100,42 @@ static void grid_shape(const ViewConsts& v, TileShape tile, uint32_t& blocks_x, 
   nblocks = blocks_x * blocks_y;
 }
 
+template <class L, bool kStats>
+static void launch_dense_shade(const DenseWorld& w, const ViewConsts& v, const ShadeConsts& sc,
+                               const FrameOut& out, uint32_t log2_w, uint32_t blocks_x, dim3 grid,
+                               hipStream_t stream) {
+  switch (shade_mode(sc.flags)) {
+    case kShadeLambertGrad:
+      hipLaunchKernelGGL((k_render_dense<L, kStats, kShadeLambertGrad>), grid, dim3(256), 0,
+                         stream, w, v, sc, out, log2_w, blocks_x);
+      break;
+    case kShadeLambert:
+      hipLaunchKernelGGL((k_render_dense<L, kStats, kShadeLambert>), grid, dim3(256), 0, stream, w,
+                         v, sc, out, log2_w, blocks_x);
+      break;
+    default:
+      hipLaunchKernelGGL((k_render_dense<L, kStats, kShadeRef>), grid, dim3(256), 0, stream, w, v,
+                         sc, out, log2_w, blocks_x);
+  }
+}
+
 hipError_t launch_render_dense(const DenseWorld& w, uint32_t traversal, const ViewConsts& v,
                                const ShadeConsts& sc, const FrameOut& out, TileShape tile,
                                hipStream_t stream) {
   uint32_t blocks_x, nblocks;
   grid_shape(v, tile, blocks_x, nblocks);
-  dim3 grid(nblocks), block(256);
+  const dim3 grid(nblocks);
   const bool stats = out.stats != nullptr;
   if (traversal == VT_TRAVERSAL_BITMASK) {
     if (stats)
-      hipLaunchKernelGGL((k_render_dense<BitmaskLookup, true>), grid, block, 0, stream, w, v, sc,
-                         out, tile.log2_w, blocks_x);
+      launch_dense_shade<BitmaskLookup, true>(w, v, sc, out, tile.log2_w, blocks_x, grid, stream);
     else
-      hipLaunchKernelGGL((k_render_dense<BitmaskLookup, false>), grid, block, 0, stream, w, v, sc,
-                         out, tile.log2_w, blocks_x);
+      launch_dense_shade<BitmaskLookup, false>(w, v, sc, out, tile.log2_w, blocks_x, grid, stream);
   } else {
     if (stats)
-      hipLaunchKernelGGL((k_render_dense<FlatLookup, true>), grid, block, 0, stream, w, v, sc, out,
-                         tile.log2_w, blocks_x);
+      launch_dense_shade<FlatLookup, true>(w, v, sc, out, tile.log2_w, blocks_x, grid, stream);
     else
-      hipLaunchKernelGGL((k_render_dense<FlatLookup, false>), grid, block, 0, stream, w, v, sc,
-                         out, tile.log2_w, blocks_x);
+      launch_dense_shade<FlatLookup, false>(w, v, sc, out, tile.log2_w, blocks_x, grid, stream);
   }
   return hipGetLastError();
 }
