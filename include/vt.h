@@ -76,6 +76,11 @@ typedef struct vt_camera {
                                       /* gradients of sum |shaded - target|^2 */
                                       /* (frame.target) into grad_albedo,     */
                                       /* grad_sun and loss (recompute mode)   */
+#define VT_FLAG_PRESENT (1u << 4)     /* shaded = the presented colour: the   */
+                                      /* composite through the crosshair and  */
+                                      /* vignette of blit.fragment.glsl:32-35 */
+                                      /* (not with LAMBERT); frame.display,   */
+                                      /* when set, gets it as RGBA8           */
 
 /* traversal storage variants for dense worlds (identical hits by contract) */
 #define VT_TRAVERSAL_FLAT 0u    /* one uint8 gather per executed step        */
@@ -145,6 +150,9 @@ typedef struct vt_frame {
   uint32_t* steps;       /* W*H: primary steps | shadow steps << 16           */
   unsigned long long* stats; /* VT_STAT_COUNT counters, accumulated (+=)      */
   const float* target;   /* W*H*4 input: VT_FLAG_GRAD_L2 target image (RGB)   */
+  uint32_t* display;     /* W*H RGBA8 (R in the low byte): VT_FLAG_PRESENT    */
+                         /* colour stored as an rgba8 framebuffer does        */
+                         /* (clamp to [0,1], round to nearest)                */
 } vt_frame;
 
 /* ---- context ------------------------------------------------------------ */
@@ -192,6 +200,17 @@ int vt_world_info(const vt_world* world, uint32_t* dim, size_t* device_bytes);
  * (blit.fragment.glsl:30-31), fused into one launch.                      */
 int vt_render(vt_ctx* ctx, const vt_world* world, const vt_camera* camera,
               const vt_render_params* params, const vt_frame* frame, void* stream);
+
+/* ---- present pass ---------------------------------------------------------- */
+/* The blit pass after the composite (blit.fragment.glsl:32-35), as a
+ * stand-alone streaming kernel over a composite image: for pixel (x, y) at
+ * texPos = ((x+0.5)/W, (y+0.5)/H) (pixel centres of the full-screen quad,
+ * blit.vertex.glsl:5-15), colour = mix(c, (1,1,1,0.4), 0.5) within 0.002 of
+ * the centre (crosshair), then * pow(u(1-u) * v(1-v) * 15, 0.6 * 0.3)
+ * (vignette), with (u, v) = texPos.  out (RGBA32F) and/or display (RGBA8) may be NULL, not both.
+ * vt_render with VT_FLAG_PRESENT does the same inside the render kernel. */
+int vt_present(vt_ctx* ctx, const float* composite, uint32_t width, uint32_t height, float* out,
+               uint32_t* display, void* stream);
 
 /* ---- differentiable Lambert shading ---------------------------------------- */
 /* Project-defined (no reference counterpart: the reference shades with the

@@ -327,6 +327,49 @@ typedef struct job {
   pthread_mutex_t lock;
 } job;
 
+
+/* ------------------------------------------------------------------ */
+/* present pass, blit.fragment.glsl:15-21,32-35                        */
+/* ------------------------------------------------------------------ */
+
+/* Stores into an rgba8 target: clamp to [0,1], round to nearest. */
+static uint32_t pack_unorm8(const float c[4]) {
+  uint32_t r = 0;
+  for (int i = 0; i < 4; i++) {
+    float k = fmin_(fmax_(c[i], 0.0f), 1.0f);
+    r |= (uint32_t)floorf(k * 255.0f + 0.5f) << (8 * i);
+  }
+  return r;
+}
+
+/* texPos = the pixel centre of the full-screen quad (blit.vertex.glsl:5-15). */
+static void present_px(float c[4], uint32_t x, uint32_t y, uint32_t W, uint32_t H) {
+  float tp[2] = {((float)x + 0.5f) / (float)W, ((float)y + 0.5f) / (float)H};
+  float d0 = tp[0] - 0.5f, d1 = tp[1] - 0.5f;
+  float d2 = d0 * d0;
+  d2 = d2 + d1 * d1;
+  if (sqrtf(d2) <= 0.002f) { /* :33 mix(color, vec4(1,1,1,0.4), 0.5) */
+    const float w[4] = {1.0f, 1.0f, 1.0f, 0.4f};
+    for (int i = 0; i < 4; i++) c[i] = c[i] * (1.0f - 0.5f) + w[i] * 0.5f;
+  }
+  /* vignetteEffect (:18-21): uv *= 1 - uv; pow(uv.x*uv.y*15, 0.6*0.3) */
+  float ux = tp[0] * (1.0f - tp[0]), uy = tp[1] * (1.0f - tp[1]);
+  float g = powf(ux * uy * 15.0f, 0.6f * 0.3f);
+  for (int i = 0; i < 4; i++) c[i] = g * c[i];
+}
+
+void or_present(const float* composite, uint32_t W, uint32_t H, float* out, uint32_t* display) {
+  for (uint32_t y = 0; y < H; y++)
+    for (uint32_t x = 0; x < W; x++) {
+      size_t i = (size_t)y * W + x;
+      float c[4];
+      memcpy(c, composite + 4 * i, sizeof c);
+      present_px(c, x, y, W, H);
+      if (out) memcpy(out + 4 * i, c, sizeof c);
+      if (display) display[i] = pack_unorm8(c);
+    }
+}
+
 static void render_pixel(job* J, uint32_t px, uint32_t py, uint64_t st[8]) {
   const or_world* w = J->w;
   const or_params* p = J->p;
@@ -457,6 +500,10 @@ static void render_pixel(job* J, uint32_t px, uint32_t py, uint64_t st[8]) {
       out[3] = color[3] + illum[3] * 1.0f;
     } else {
       memcpy(out, color, sizeof out);
+    }
+    if ((p->flags & 16u) && !lambert) {
+      present_px(out, px, py, p->width, p->height);
+      if (f->display) f->display[pix] = pack_unorm8(out);
     }
     memcpy(f->shaded + 4 * pix, out, sizeof out);
   }

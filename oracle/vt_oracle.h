@@ -101,13 +101,14 @@ typedef struct or_frame {
   uint32_t* steps;   /* W*H */
   float* tie_margin; /* W*H: min of primary and shadow margins */
   uint64_t* stats;   /* 8 counters, accumulated */
+  uint32_t* display; /* W*H RGBA8 of the presented colour (flags bit4) */
 } or_frame;
 
 typedef struct or_params {
   uint32_t width, height;
   int32_t max_steps_primary, max_steps_shadow;
   float sun[3];
-  uint32_t flags;     /* bit0 GBUF_UNORM8, bit1 LAMBERT */
+  uint32_t flags;     /* bit0 GBUF_UNORM8, bit1 LAMBERT, bit4 PRESENT */
   uint32_t traversal; /* 0 flat, 1 bitmask (dense worlds: memory counters only) */
   float ambient;
   const float* albedo; /* LAMBERT: dim^3*3 floats */
@@ -116,6 +117,11 @@ typedef struct or_params {
 /* Render rows [y0, y1) of the frame with `threads` POSIX threads. */
 void or_render(const or_world* w, const or_camera* cam, const or_params* p,
                const or_frame* f, uint32_t y0, uint32_t y1, int threads);
+
+/* Present pass of one composite image (blit.fragment.glsl:32-35): crosshair
+ * and vignette at texPos = ((x+0.5)/W, (y+0.5)/H); out RGBA32F and/or
+ * display RGBA8 (clamp, round to nearest, R in the low byte), either NULL. */
+void or_present(const float* composite, uint32_t W, uint32_t H, float* out, uint32_t* display);
 
 /* Lambert x visibility shading from hit records (project-defined):
  * rgb = albedo[voxel] * (ambient + max(dot(n, sun/|sun|), 0) * vis),

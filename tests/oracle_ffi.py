@@ -31,7 +31,8 @@ class OrHit(C.Structure):
 class OrFrame(C.Structure):
     _fields_ = [("shaded", C.c_void_p), ("hits", C.c_void_p), ("color", C.c_void_p),
                 ("normal", C.c_void_p), ("position", C.c_void_p), ("illumination", C.c_void_p),
-                ("steps", C.c_void_p), ("tie_margin", C.c_void_p), ("stats", C.c_void_p)]
+                ("steps", C.c_void_p), ("tie_margin", C.c_void_p), ("stats", C.c_void_p),
+                ("display", C.c_void_p)]
 
 
 class OrParams(C.Structure):
@@ -66,6 +67,7 @@ def lib():
         L.or_sky.argtypes = [f3, f3, f3]
         L.or_render.argtypes = [C.POINTER(OrWorld), C.POINTER(OrCamera), C.POINTER(OrParams),
                                 C.POINTER(OrFrame), C.c_uint32, C.c_uint32, C.c_int]
+        L.or_present.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]
         L.or_shade_lambert.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, f3, C.c_float,
                                        C.c_void_p]
         L.or_lambert_grad64.restype = C.c_double
@@ -115,7 +117,7 @@ def trace(world, o, d, max_steps, path_cap=0):
 
 
 def render(world, cam, width, height, max_primary, max_shadow, sun, flags=0, traversal=0,
-           ambient=0.0, albedo=None, rows=None, threads=8, gbuffer=True):
+           ambient=0.0, albedo=None, rows=None, threads=8, gbuffer=True, display=False):
     """Full (or row-range) oracle render; returns a dict of numpy arrays."""
     p = OrParams(width, height, max_primary, max_shadow, fvec(sun), flags, traversal,
                  float(ambient), None if albedo is None else albedo.ctypes.data)
@@ -126,13 +128,26 @@ def render(world, cam, width, height, max_primary, max_shadow, sun, flags=0, tra
         "tie_margin": np.zeros((height, width), np.float32),
         "stats": np.zeros(8, np.uint64),
     }
+    if display:
+        out["display"] = np.zeros((height, width), np.uint32)
     if gbuffer:
         for k in ("color", "normal", "position", "illumination"):
             out[k] = np.zeros((height, width, 4), np.float32)
     ptr = lambda k: out[k].ctypes.data if k in out else None
     f = OrFrame(ptr("shaded"), ptr("hits"), ptr("color"), ptr("normal"), ptr("position"),
-                ptr("illumination"), ptr("steps"), ptr("tie_margin"), ptr("stats"))
+                ptr("illumination"), ptr("steps"), ptr("tie_margin"), ptr("stats"),
+                ptr("display"))
     y0, y1 = (0, height) if rows is None else rows
     oc = camera(cam)
     lib().or_render(C.byref(world.s), C.byref(oc), C.byref(p), C.byref(f), y0, y1, threads)
     return out
+
+
+def present(composite):
+    """Oracle present pass over an (H, W, 4) float32 composite: (out, display)."""
+    c = np.ascontiguousarray(composite, dtype=np.float32)
+    H, W = c.shape[:2]
+    out = np.zeros_like(c)
+    disp = np.zeros((H, W), np.uint32)
+    lib().or_present(c.ctypes.data, W, H, out.ctypes.data, disp.ctypes.data)
+    return out, disp

@@ -144,3 +144,85 @@ def test_brickmap_embedding_c_two_levels(ctx, c0, flags):
     compare(g, o, stats=False, label=f"brickmap embedding C flags={flags}")
     assert (g["hits"][..., 0] != 0xFFFFFFFF).mean() > 0.3
     w.close()
+
+
+# ---------------------------------------------------------------------------
+# present pass (blit.fragment.glsl:32-35): crosshair + vignette + rgba8 store
+# ---------------------------------------------------------------------------
+
+def _display_close(g, o, label):
+    """RGBA8 bytes equal, except +-1 where the GPU's powf and the host's
+    differ across a rounding boundary (counted, at most 1e-4 of pixels)."""
+    gb = g.view(np.uint8).astype(np.int16)
+    ob = o.view(np.uint8).astype(np.int16)
+    diff = np.abs(gb - ob)
+    assert diff.max() <= 1, f"{label}: display bytes differ by {diff.max()}"
+    assert np.count_nonzero(diff) <= max(1, diff.size // 10000), f"{label}: {np.count_nonzero(diff)}"
+
+
+@pytest.mark.parametrize("flags", [vt.VT_FLAG_PRESENT, vt.VT_FLAG_PRESENT | vt.VT_FLAG_GBUF_UNORM8])
+def test_c0_present_fused(ctx, c0, flags):
+    import torch
+    grid, pal = c0
+    w = vt.World.dense(ctx, grid, pal)
+    cam, W, H = S.views("c0")[0]
+    f = vt.Frame.alloc(W, H, display=True)
+    print(f"[parity] c0 present flags={flags}", flush=True)
+    vt.render(ctx, w, cam, f, vt.params(W, H, 512, 512, S.SUN, flags))
+    torch.cuda.synchronize()
+    o = O.render(O.DenseWorld(grid, pal), cam, W, H, 512, 512, S.SUN, flags=flags, gbuffer=False,
+                 display=True)
+    assert np.array_equal(f.hits.cpu().numpy().view(np.uint32), o["hits"])
+    err = np.max(np.abs(f.shaded.cpu().numpy() - o["shaded"]))
+    assert err <= RGB_TOL, f"presented colour max-abs {err}"
+    _display_close(f.display.cpu().numpy().view(np.uint32), o["display"], "c0 display")
+    # the present pass of the plain composite equals the fused result
+    f2 = vt.Frame.alloc(W, H)
+    vt.render(ctx, w, cam, f2, vt.params(W, H, 512, 512, S.SUN, flags & ~vt.VT_FLAG_PRESENT))
+    out = torch.empty_like(f2.shaded)
+    disp = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    vt.present(ctx, f2.shaded, out, disp)
+    torch.cuda.synchronize()
+    assert torch.equal(out, f.shaded) and torch.equal(disp, f.display)
+    w.close()
+
+
+def test_present_standalone_1080p(ctx):
+    import torch
+    rng = np.random.default_rng(5)
+    comp = rng.uniform(-0.2, 1.2, (1080, 1920, 4)).astype(np.float32)
+    c = torch.from_numpy(comp).cuda()
+    out = torch.empty_like(c)
+    disp = torch.empty((1080, 1920), dtype=torch.int32, device="cuda")
+    print("[parity] vt_present 1920x1080", flush=True)
+    vt.present(ctx, c, out, disp)
+    torch.cuda.synchronize()
+    oo, od = O.present(comp)
+    assert np.max(np.abs(out.cpu().numpy() - oo)) <= RGB_TOL
+    _display_close(disp.cpu().numpy().view(np.uint32), od, "1080p display")
+    # display only
+    d2 = torch.zeros_like(disp)
+    vt.present(ctx, c, None, d2)
+    torch.cuda.synchronize()
+    assert torch.equal(d2, disp)
+
+
+def test_present_argument_errors(ctx):
+    import torch
+    c = torch.zeros((4, 4, 4), device="cuda")
+    with pytest.raises(vt.VtError) as e:
+        vt.present(ctx, c, None, None)
+    assert e.value.status == vt._native.VT_ERR_NULL_POINTER
+    grid = S.generate(S.spec("c0"))
+    w = vt.World.dense(ctx, grid, S.palette())
+    cam, W, H = S.views("c0")[0]
+    f = vt.Frame.alloc(W, H, display=True)
+    with pytest.raises(vt.VtError) as e:  # display without VT_FLAG_PRESENT
+        vt.render(ctx, w, cam, f, vt.params(W, H, 512, 512, S.SUN, 0))
+    assert e.value.status == vt._native.VT_ERR_INVALID_ARGUMENT
+    with pytest.raises(vt.VtError) as e:  # PRESENT with LAMBERT
+        alb = torch.zeros(128 ** 3 * 3, device="cuda")
+        vt.render(ctx, w, cam, f, vt.params(W, H, 512, 512, S.SUN,
+                                             vt.VT_FLAG_PRESENT | vt.VT_FLAG_LAMBERT, 0, 0.2, alb))
+    assert e.value.status == vt._native.VT_ERR_INVALID_ARGUMENT
+    w.close()

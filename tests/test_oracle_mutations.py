@@ -37,6 +37,16 @@ MUTATIONS = [
     ("pool in-block index x+8z+64y",
      "(uint64_t)(((bz % 8) * 8 + (by % 8)) * 8);", "(uint64_t)(((by % 8) * 8 + (bz % 8)) * 8);",
      [P.test_brickmap_layout_renders_like_dense]),
+    ("present texPos without +0.5",
+     "float tp[2] = {((float)x + 0.5f) / (float)W, ((float)y + 0.5f) / (float)H};",
+     "float tp[2] = {(float)x / (float)W, (float)y / (float)H};",
+     [P.test_present_crosshair_known_sizes]),
+    ("vignette exponent 0.6", "float g = powf(ux * uy * 15.0f, 0.6f * 0.3f);",
+     "float g = powf(ux * uy * 15.0f, 0.6f);",
+     [lambda: P.test_present_vignette_closed_form(333, 77)]),
+    ("rgba8 byte order BGRA", "r |= (uint32_t)floorf(k * 255.0f + 0.5f) << (8 * i);",
+     "r |= (uint32_t)floorf(k * 255.0f + 0.5f) << (8 * (i < 3 ? 2 - i : 3));",
+     [P.test_present_rgba8_store]),
 ]
 
 

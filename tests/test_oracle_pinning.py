@@ -25,6 +25,12 @@ is pinned here by checks that do not retype its formula:
                                       reference's missing t >= 0 check.
   SkyDome2 (camera.glsl:11-19)       float64 closed form.
   composite (blit.fragment.glsl:30)  recomputed from the oracle's G-buffer.
+  present (blit.fragment.glsl:32-35) crosshair pixel set = float64 brute
+                                      force of |texPos - 0.5| <= 0.002 at
+                                      pixel centres; vignette = float64
+                                      closed form (and centre symmetry);
+                                      rgba8 store: clamp, round-to-nearest
+                                      known answers, R in the low byte.
   Lambert shading + FP64 gradient    float64 recomputation; central finite
                                       differences of the FP64 loss.
   chunk table / block pool / atlas   a brickmap world built like
@@ -563,3 +569,69 @@ def test_two_level_traversal_hits_solid_subvoxels():
     cheb = np.maximum(np.maximum(np.abs(vd % 128 - sx), np.abs(vd // 128 % 128 - sy)),
                       np.abs(vd // 16384 - sz))
     assert np.mean(cheb[ok] <= 1) > 0.99
+
+
+# ---------------------------------------------------------------------------
+# present pass
+# ---------------------------------------------------------------------------
+
+def _crosshair64(W, H):
+    x = (np.arange(W) + 0.5) / W - 0.5
+    y = (np.arange(H) + 0.5) / H - 0.5
+    d = np.sqrt(x[None, :] ** 2 + y[:, None] ** 2)
+    return d, d <= 0.002
+
+
+@pytest.mark.parametrize("W,H", [(1920, 1080), (256, 256), (1000, 1000), (999, 1001), (7, 5)])
+def test_present_crosshair_set(W, H):
+    out, _ = O.present(np.zeros((H, W, 4), np.float32))
+    got = out[..., 0] > 0  # zero composite: only crosshair pixels turn grey
+    d, exp = _crosshair64(W, H)
+    ambiguous = np.abs(d - 0.002) < 1e-7
+    assert np.array_equal(got[~ambiguous], exp[~ambiguous])
+    # the crosshair colour is mix(0, (1,1,1,0.4), 0.5) times the vignette
+    g = out[got][:, 3] / 0.2
+    assert np.allclose(out[got][:, 0], 0.5 * g, rtol=1e-6)
+
+
+def test_present_crosshair_known_sizes():
+    # 1000x1000: radius 0.002 = 2 pixels, centres at offsets +-0.5, +-1.5 px
+    out, _ = O.present(np.zeros((1000, 1000, 4), np.float32))
+    ys, xs = np.nonzero(out[..., 0] > 0)
+    assert (xs.min(), xs.max(), ys.min(), ys.max()) == (498, 501, 498, 501)
+    assert len(xs) == 12  # 4x4 square minus the 4 corners (|(1.5,1.5)| > 2)
+    # a 7x5 image has no pixel centre within 0.002 of the middle except (3, 2)
+    out, _ = O.present(np.zeros((5, 7, 4), np.float32))
+    assert list(zip(*np.nonzero(out[..., 0] > 0))) == [(2, 3)]
+
+
+@pytest.mark.parametrize("W,H", [(1920, 1080), (333, 77)])
+def test_present_vignette_closed_form(W, H):
+    comp = np.ones((H, W, 4), np.float32)
+    out, _ = O.present(comp)
+    u = (np.arange(W) + 0.5) / W
+    v = (np.arange(H) + 0.5) / H
+    g = (u[None, :] * (1 - u[None, :]) * v[:, None] * (1 - v[:, None]) * 15.0) ** 0.18
+    _, cross = _crosshair64(W, H)
+    assert np.max(np.abs(out[~cross][:, 0] - g[~cross])) <= 2e-6
+    # the pixel-centre mapping is symmetric about the image centre
+    assert np.max(np.abs(out[..., 1] - out[::-1, ::-1, 1])) <= 2e-6
+    # centre of an even image: u = v = 0.5 +- 1/(2W) -> g ~ (15/16)^0.18
+    assert abs(out[H // 2, W // 2 - 3, 2] - (15 / 16) ** 0.18) < 1e-4
+
+
+def test_present_rgba8_store():
+    comp = np.zeros((3, 4, 4), np.float32)
+    comp[0, 0] = [-1.0, 2.0, 0.5, 1.0]
+    comp[0, 1] = [np.nan, 0.25, 0.75, 0.0]
+    out, disp = O.present(comp)
+    d = disp[0, 0]
+    r, g, b, a = d & 255, (d >> 8) & 255, (d >> 16) & 255, d >> 24
+    assert r == 0 and g == 255  # clamped
+    exp_b = math.floor(f32(f32(out[0, 0, 2]) * 255.0) + 0.5)
+    assert b == exp_b and a == math.floor(f32(out[0, 0, 3] * 255.0) + 0.5)
+    # half-way cases round up: 0.5 * 255 = 127.5 -> 128
+    comp2 = np.full((1, 1, 4), 0.5, np.float32)
+    out2, disp2 = O.present(comp2)
+    k = out2[0, 0, 0]
+    assert disp2[0, 0] & 255 == int(np.floor(np.float32(k) * np.float32(255) + np.float32(0.5)))

@@ -124,8 +124,11 @@ int check_params(const vt_render_params& p, const vt_world* world, const char* w
       p.max_steps_shadow > 65535)
     return fail(VT_ERR_INVALID_ARGUMENT, "%s: step caps must lie in [0, 65535]", who);
   if (p.flags & ~(VT_FLAG_GBUF_UNORM8 | VT_FLAG_LAMBERT | VT_FLAG_ALBEDO_COMPACT |
-                  VT_FLAG_GRAD_L2))
+                  VT_FLAG_GRAD_L2 | VT_FLAG_PRESENT))
     return fail(VT_ERR_INVALID_ARGUMENT, "%s: unknown flags 0x%x", who, p.flags);
+  if ((p.flags & VT_FLAG_PRESENT) && (p.flags & VT_FLAG_LAMBERT))
+    return fail(VT_ERR_INVALID_ARGUMENT, "%s: VT_FLAG_PRESENT applies to the composite, not "
+                "to VT_FLAG_LAMBERT", who);
   if ((p.flags & (VT_FLAG_ALBEDO_COMPACT | VT_FLAG_GRAD_L2)) && !(p.flags & VT_FLAG_LAMBERT))
     return fail(VT_ERR_INVALID_ARGUMENT, "%s: ALBEDO_COMPACT / GRAD_L2 need VT_FLAG_LAMBERT", who);
   if ((p.flags & VT_FLAG_ALBEDO_COMPACT) && (world->kind != WorldKind::Dense || !world->prefix))
@@ -148,12 +151,19 @@ int check_params(const vt_render_params& p, const vt_world* world, const char* w
   return VT_OK;
 }
 
-int check_frame(const vt_frame* frame, const char* who) {
+int check_frame(const vt_frame* frame, uint32_t flags, const char* who) {
+  if ((flags & VT_FLAG_GRAD_L2) && !frame->target)
+    return fail(VT_ERR_NULL_POINTER, "%s: VT_FLAG_GRAD_L2 needs frame->target", who);
+  if (frame->display && !(flags & VT_FLAG_PRESENT))
+    return fail(VT_ERR_INVALID_ARGUMENT, "%s: frame->display needs VT_FLAG_PRESENT", who);
+  if (frame->display && !frame->shaded)
+    return fail(VT_ERR_NULL_POINTER, "%s: frame->display is written with frame->shaded", who);
   const void* f16[] = {frame->shaded, frame->color, frame->normal, frame->position,
                        frame->illumination, frame->target};
   for (const void* q : f16)
     if (q && !aligned(q, 16)) return fail(VT_ERR_MISALIGNED, "%s: image not 16B aligned", who);
   if ((frame->hits && !aligned(frame->hits, 4)) || (frame->steps && !aligned(frame->steps, 4)) ||
+      (frame->display && !aligned(frame->display, 4)) ||
       (frame->stats && !aligned(frame->stats, 8)))
     return fail(VT_ERR_MISALIGNED, "%s: hits/steps/stats misaligned", who);
   return VT_OK;
@@ -194,6 +204,7 @@ FrameOut frame_out(const vt_frame* frame) {
   o.steps = frame->steps;
   o.stats = frame->stats;
   o.target = reinterpret_cast<const float4*>(frame->target);
+  o.display = frame->display;
   return o;
 }
 
@@ -375,9 +386,7 @@ int vt_render(vt_ctx* ctx, const vt_world* world, const vt_camera* camera,
   const vt_render_params& p = *params;
   int st = vt::check_image_size(p.width, p.height, "vt_render");
   if (st == VT_OK) st = vt::check_params(p, world, "vt_render");
-  if (st == VT_OK) st = vt::check_frame(frame, "vt_render");
-  if (st == VT_OK && (p.flags & VT_FLAG_GRAD_L2) && !frame->target)
-    st = fail(VT_ERR_NULL_POINTER, "vt_render: VT_FLAG_GRAD_L2 needs frame->target");
+  if (st == VT_OK) st = vt::check_frame(frame, p.flags, "vt_render");
   if (st != VT_OK) return st;
   VT_HIP(hipSetDevice(ctx->device));
   const vt::ViewConsts v = vt::view_consts(camera, p.width, p.height);

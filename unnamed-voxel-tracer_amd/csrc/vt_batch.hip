@@ -170,10 +170,8 @@ int vt_batch_create(vt_ctx* ctx, const vt_world* world, const vt_view* views, ui
   for (uint32_t i = 0; i < n_views; i++) {
     const vt_view& v = views[i];
     if ((st = vt::check_image_size(v.width, v.height, "vt_batch_create")) != VT_OK ||
-        (st = vt::check_frame(&v.frame, "vt_batch_create")) != VT_OK)
+        (st = vt::check_frame(&v.frame, p.flags, "vt_batch_create")) != VT_OK)
       return st;
-    if ((p.flags & VT_FLAG_GRAD_L2) && !v.frame.target)
-      return fail(VT_ERR_NULL_POINTER, "vt_batch_create: view %u has no target", i);
     vt::BatchView& b = table[i];
     std::memset(&b, 0, sizeof b);
     b.v = vt::view_consts(&v.camera, v.width, v.height);

@@ -75,6 +75,7 @@ struct alignas(8) FrameOut {
   uint32_t* steps;
   unsigned long long* stats;
   const float4* target;  // VT_FLAG_GRAD_L2 input
+  uint32_t* display;     // VT_FLAG_PRESENT RGBA8 output
 };
 
 // Wave tile: 64 lanes cover tile_w x (64 / tile_w) pixels; a 256-thread
@@ -89,7 +90,7 @@ BrickmapWorld brickmap_view(const vt_world* w);
 // Host-side validation and translation of the C-ABI structs.
 int check_image_size(uint32_t W, uint32_t H, const char* who);
 int check_params(const vt_render_params& p, const vt_world* world, const char* who);
-int check_frame(const vt_frame* frame, const char* who);
+int check_frame(const vt_frame* frame, uint32_t flags, const char* who);
 ViewConsts view_consts(const vt_camera* camera, uint32_t W, uint32_t H);
 ShadeConsts shade_consts(const vt_render_params& p);
 FrameOut frame_out(const vt_frame* frame);
@@ -110,6 +111,8 @@ hipError_t launch_compact_prefix(const uint64_t* l0, uint64_t nwords, uint32_t* 
                                  uint32_t* scratch, hipStream_t stream);
 
 // Lambert shading / backward from stored hit records.
+hipError_t launch_present(const float4* composite, uint32_t W, uint32_t H, float4* out,
+                          uint32_t* display, hipStream_t stream);
 hipError_t launch_lambert_shade(const DenseWorld& w, const ShadeConsts& sc, const uint32_t* hits,
                                 uint32_t npix, float4* out, hipStream_t stream);
 hipError_t launch_lambert_backward(const DenseWorld& w, const ShadeConsts& sc,

@@ -240,6 +240,10 @@ __device__ __forceinline__ void write_pixel(const ViewConsts& v, const ShadeCons
     } else {
       s = col;
     }
+    if (sc.flags & VT_FLAG_PRESENT) {
+      s = present_pixel(s, px, py, v.W, v.H);
+      if (out.display) out.display[pix] = pack_unorm8(s);
+    }
     out.shaded[pix] = s;
   }
 }

@@ -47,6 +47,37 @@ __device__ __forceinline__ float unorm8(float x) {
   return b / 255.0f;
 }
 
+// RGBA8 framebuffer store of a colour: per channel clamp to [0,1], round to
+// nearest (the unorm8 rule), R in the low byte.
+__device__ __forceinline__ uint32_t pack_unorm8(float4 c) {
+  const float v[4] = {c.x, c.y, c.z, c.w};
+  uint32_t r = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const float k = fminf(fmaxf(v[i], 0.0f), 1.0f);
+    r |= (uint32_t)floorf(k * 255.0f + 0.5f) << (8 * i);
+  }
+  return r;
+}
+
+// Present pass of one pixel (blit.fragment.glsl:32-35): texPos is the pixel
+// centre of the full-screen quad, ((x+0.5)/W, (y+0.5)/H).
+__device__ __forceinline__ float4 present_pixel(float4 c, uint32_t x, uint32_t y, uint32_t W,
+                                                uint32_t H) {
+  const float u = ((float)x + 0.5f) / (float)W;
+  const float v = ((float)y + 0.5f) / (float)H;
+  const float du = u - 0.5f, dv = v - 0.5f;
+  float d2 = du * du;
+  d2 = d2 + dv * dv;
+  if (sqrtf(d2) <= 0.002f) {  // crosshair: mix(color, vec4(1,1,1,0.4), 0.5)
+    c = make_float4(c.x * 0.5f + 0.5f, c.y * 0.5f + 0.5f, c.z * 0.5f + 0.5f,
+                    c.w * 0.5f + 0.4f * 0.5f);
+  }
+  const float a = u * (1.0f - u), b = v * (1.0f - v);
+  const float g = powf((a * b) * 15.0f, 0.6f * 0.3f);  // vignette
+  return make_float4(g * c.x, g * c.y, g * c.z, g * c.w);
+}
+
 // normals[] (map.glsl:72-79), face id 1..6
 __device__ __forceinline__ void face_normal(int face, float& nx, float& ny, float& nz) {
   nx = face == 1 ? -1.0f : (face == 2 ? 1.0f : 0.0f);

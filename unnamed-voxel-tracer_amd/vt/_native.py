@@ -36,6 +36,7 @@ VT_FLAG_GBUF_UNORM8 = 1 << 0
 VT_FLAG_LAMBERT = 1 << 1
 VT_FLAG_ALBEDO_COMPACT = 1 << 2
 VT_FLAG_GRAD_L2 = 1 << 3
+VT_FLAG_PRESENT = 1 << 4
 VT_UPSTREAM_GRADIENT = 0
 VT_UPSTREAM_L2_TARGET = 1
 VT_TRAVERSAL_FLAT = 0
@@ -63,7 +64,7 @@ class Frame(C.Structure):
     _fields_ = [("shaded", C.c_void_p), ("hits", C.c_void_p), ("color", C.c_void_p),
                 ("normal", C.c_void_p), ("position", C.c_void_p),
                 ("illumination", C.c_void_p), ("steps", C.c_void_p), ("stats", C.c_void_p),
-                ("target", C.c_void_p)]
+                ("target", C.c_void_p), ("display", C.c_void_p)]
 
 
 class View(C.Structure):
@@ -80,8 +81,8 @@ VT_TILE_8X8, VT_TILE_16X4, VT_TILE_32X2 = 3, 4, 5
 
 assert C.sizeof(Camera) == 96
 assert C.sizeof(RenderParams) == 72
-assert C.sizeof(Frame) == 72
-assert C.sizeof(View) == 184
+assert C.sizeof(Frame) == 80
+assert C.sizeof(View) == 192
 
 
 class VtError(RuntimeError):
@@ -116,6 +117,7 @@ def lib() -> C.CDLL:
         "vt_world_info": (i32, [vp, C.POINTER(u32), C.POINTER(C.c_size_t)]),
         "vt_render": (i32, [vp, vp, C.POINTER(Camera), C.POINTER(RenderParams),
                             C.POINTER(Frame), vp]),
+        "vt_present": (i32, [vp, vp, u32, u32, vp, vp, vp]),
         "vt_world_build_compact_index": (i32, [vp, vp, C.POINTER(C.c_uint64)]),
         "vt_lambert_shade": (i32, [vp, vp, vp, u32, C.POINTER(RenderParams), vp, vp]),
         "vt_lambert_backward": (i32, [vp, vp, vp, u32, C.POINTER(RenderParams), vp, u32, vp, vp,

@@ -110,10 +110,11 @@ class Frame:
     steps: object = None
     stats: object = None
     target: object = None
+    display: object = None
 
     @classmethod
     def alloc(cls, width, height, device=0, shaded=True, hits=True, gbuffer=False, steps=False,
-              stats=False):
+              stats=False, display=False):
         torch = _torch()
         dev = f"cuda:{device}"
         f = cls(width, height)
@@ -128,13 +129,15 @@ class Frame:
             f.steps = torch.empty((height, width), dtype=torch.int32, device=dev)
         if stats:
             f.stats = torch.zeros(8, dtype=torch.int64, device=dev)
+        if display:
+            f.display = torch.empty((height, width), dtype=torch.int32, device=dev)
         return f
 
     def struct(self) -> N.Frame:
         p = lambda t: None if t is None else t.data_ptr()
         return N.Frame(p(self.shaded), p(self.hits), p(self.color), p(self.normal),
                        p(self.position), p(self.illumination), p(self.steps), p(self.stats),
-                       p(self.target))
+                       p(self.target), p(self.display))
 
     def stats_dict(self):
         s = self.stats.cpu().numpy()
@@ -153,6 +156,15 @@ def params(width, height, max_primary=512, max_shadow=512, sun=SUN, flags=0, tra
     p.albedo, p.grad_albedo, p.grad_sun, p.loss = (ptr(albedo), ptr(grad_albedo), ptr(grad_sun),
                                                    ptr(loss))
     return p
+
+
+def present(ctx: Context, composite, out=None, display=None, stream=None):
+    """Present pass (crosshair + vignette) over an (H, W, 4) float32 composite."""
+    H, W = composite.shape[0], composite.shape[1]
+    s = current_stream() if stream is None else stream
+    ptr = lambda t: None if t is None else t.data_ptr()
+    N.check(N.lib().vt_present(ctx.h, composite.data_ptr(), W, H, ptr(out), ptr(display),
+                               C.c_void_p(s)), "vt_present")
 
 
 def build_compact_index(ctx: Context, world: World) -> int:
