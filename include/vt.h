@@ -190,6 +190,19 @@ int vt_world_create_dense_u8(vt_ctx* ctx, const uint8_t* grid, uint32_t dim,
 int vt_world_create_brickmap(vt_ctx* ctx, uint32_t dim, const uint32_t* chunks,
                              const uint32_t* pool, size_t pool_words,
                              const uint32_t* atlas_rgba8, vt_world** out);
+/* MagicaVoxel model files -> the model atlas (host side, no device work).
+ * vt_atlas_load_vox replaces VoxelModelAtlas.load_block_model
+ * (voxel.zig:94-127): every model of the file (SIZE/XYZI pairs in file
+ * order) takes the next slot k = *next_slot, its 8^3 box at
+ * ((k%32)*8, (k/32%32)*8, (k/1024%1024)*8) of atlas_rgba8 (256^3 texels,
+ * index x + 256*(y + 256*z)) is cleared and voxel (x, y, z, colour c) is
+ * written at box offset (x, z, y) with the file's RGBA palette entry c-1;
+ * *next_slot advances by the model count.  Files without an RGBA chunk are
+ * rejected (VT_ERR_UNSUPPORTED).  vt_vox_info counts the models.          */
+int vt_vox_info(const uint8_t* bytes, size_t len, uint32_t* n_models, uint32_t* version);
+int vt_atlas_load_vox(const uint8_t* bytes, size_t len, uint32_t* atlas_rgba8,
+                      uint32_t* next_slot);
+
 int vt_world_destroy(vt_world* world);
 /* dim and device bytes held by the world. */
 int vt_world_info(const vt_world* world, uint32_t* dim, size_t* device_bytes);

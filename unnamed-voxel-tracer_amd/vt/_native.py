@@ -118,6 +118,8 @@ def lib() -> C.CDLL:
         "vt_render": (i32, [vp, vp, C.POINTER(Camera), C.POINTER(RenderParams),
                             C.POINTER(Frame), vp]),
         "vt_present": (i32, [vp, vp, u32, u32, vp, vp, vp]),
+        "vt_vox_info": (i32, [vp, C.c_size_t, C.POINTER(u32), C.POINTER(u32)]),
+        "vt_atlas_load_vox": (i32, [vp, C.c_size_t, vp, C.POINTER(u32)]),
         "vt_world_build_compact_index": (i32, [vp, vp, C.POINTER(C.c_uint64)]),
         "vt_lambert_shade": (i32, [vp, vp, vp, u32, C.POINTER(RenderParams), vp, vp]),
         "vt_lambert_backward": (i32, [vp, vp, vp, u32, C.POINTER(RenderParams), vp, u32, vp, vp,
