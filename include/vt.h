@@ -203,6 +203,42 @@ int vt_vox_info(const uint8_t* bytes, size_t len, uint32_t* n_models, uint32_t* 
 int vt_atlas_load_vox(const uint8_t* bytes, size_t len, uint32_t* atlas_rgba8,
                       uint32_t* next_slot);
 
+/* ---- world edits, reads and the centre-ray pick ---------------------------- */
+/* Edits: n voxels at block coordinates xyz[3i..3i+2] (each < dim) get
+ * values[i], with the meaning of calling VoxelBrickmap.set (voxel.zig:58-61)
+ * once per edit in order: brickmap worlds take reference block words
+ * (Voxel: ty | is_solid << 28, voxel.zig:7-19) and allocate a chunk's pool
+ * block on its first edit (voxel.zig:47-56; the pool grows by doubling,
+ * gpu_block_allocator.zig:20-30); dense worlds take materials 0..255 and
+ * keep their occupancy masks (and the compact index, when built) current.
+ * Of several edits of one voxel the last wins.  The call returns after the
+ * edits are applied on `stream`; renders on other streams must be ordered
+ * after it by the caller.                                                   */
+int vt_world_set_voxels(vt_ctx* ctx, vt_world* world, const uint32_t* xyz,
+                        const uint32_t* values, uint32_t n, void* stream);
+/* Reads: values[i] = map_getVoxel of block xyz[3i..] (map.glsl:38-47), the
+ * block word for brickmap worlds (0 where the chunk is unallocated), the
+ * material for dense worlds.  Host arrays; returns when values are written. */
+int vt_world_get_voxels(vt_ctx* ctx, const vt_world* world, const uint32_t* xyz, uint32_t n,
+                        uint32_t* values, void* stream);
+
+/* terrain_edit.comp.glsl:11-24: the camera's centre ray (rayUV = 0) through
+ * the AABB entry and traceMap with max_steps (64 in the shader).  place /
+ * remove are the block cells the shader's (disabled) edit body would write:
+ * ivec3(hit_pos + normal) >> 3 and ivec3(hit_pos) >> 3.                     */
+typedef struct vt_pick_result {
+  uint32_t data;      /* HitInfo.data: RGBA8 of the hit sub-voxel, 0 = miss */
+  uint32_t face;      /* 1..6 (normals[face-1], map.glsl:72-79), 0 = miss   */
+  uint32_t steps;     /* traceMap iterations executed                       */
+  uint32_t _pad;
+  float hit_pos[3];   /* HitInfo.hit_pos in sub-voxel units; (-1,-1,-1) miss */
+  float normal[3];    /* HitInfo.normal; (0,0,0) on a miss                   */
+  int32_t place[3];
+  int32_t remove[3];
+} vt_pick_result;
+int vt_pick(vt_ctx* ctx, const vt_world* world, const vt_camera* camera, int32_t max_steps,
+            vt_pick_result* result, void* stream);
+
 int vt_world_destroy(vt_world* world);
 /* dim and device bytes held by the world. */
 int vt_world_info(const vt_world* world, uint32_t* dim, size_t* device_bytes);

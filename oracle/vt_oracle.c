@@ -608,3 +608,44 @@ double or_lambert_grad64(const uint32_t* hits, uint32_t npix, const float* albed
   }
   return loss;
 }
+
+/* ------------------------------------------------------------------ */
+/* centre-ray pick, terrain_edit.comp.glsl:11-24                        */
+/* ------------------------------------------------------------------ */
+
+void or_pick(const or_world* w, const or_camera* cam, int max_steps, or_pick_result* out) {
+  /* rayDir = normalize(C_view * vec4(rayUV = 0, 1.0, 1.0)).xyz */
+  const float* M = cam->view;
+  float r[4];
+  for (int i = 0; i < 4; i++) {
+    float acc = M[0 * 4 + i] * 0.0f;
+    acc = acc + M[1 * 4 + i] * 0.0f;
+    acc = acc + M[2 * 4 + i] * 1.0f;
+    acc = acc + M[3 * 4 + i] * 1.0f;
+    r[i] = acc;
+  }
+  float q = r[0] * r[0];
+  q = q + r[1] * r[1];
+  q = q + r[2] * r[2];
+  q = q + r[3] * r[3];
+  float l = sqrtf(q);
+  float d[3] = {r[0] / l, r[1] / l, r[2] / l};
+  float o[3] = {cam->position[0], cam->position[1], cam->position[2]}, e[3];
+  or_entry_origin(o, d, w->dim, e);
+  or_hit h;
+  memset(out, 0, sizeof *out);
+  int hit = or_trace_map(w, e, d, max_steps, &h, NULL, 0);
+  out->steps = (uint32_t)h.steps;
+  if (hit) {
+    out->data = h.data;
+    out->face = (uint32_t)h.face;
+    memcpy(out->hit_pos, h.hit_pos, sizeof out->hit_pos);
+    memcpy(out->normal, h.normal, sizeof out->normal);
+  } else { /* map.glsl:167 */
+    out->hit_pos[0] = out->hit_pos[1] = out->hit_pos[2] = -1.0f;
+  }
+  for (int i = 0; i < 3; i++) { /* :21 ivec3(hit_pos + normal) >> 3, :23 ivec3(hit_pos) >> 3 */
+    out->place[i] = f2i(out->hit_pos[i] + out->normal[i]) >> 3;
+    out->remove[i] = f2i(out->hit_pos[i]) >> 3;
+  }
+}

@@ -118,6 +118,16 @@ typedef struct or_params {
 void or_render(const or_world* w, const or_camera* cam, const or_params* p,
                const or_frame* f, uint32_t y0, uint32_t y1, int threads);
 
+/* terrain_edit.comp.glsl:11-24: the centre ray (rayUV = 0) through the AABB
+ * entry, traceMap(.., max_steps); place / remove = the edit cells
+ * ivec3(hit_pos + normal) >> 3 and ivec3(hit_pos) >> 3. */
+typedef struct or_pick_result {
+  uint32_t data, face, steps, _pad;
+  float hit_pos[3], normal[3];
+  int32_t place[3], remove[3];
+} or_pick_result;
+void or_pick(const or_world* w, const or_camera* cam, int max_steps, or_pick_result* out);
+
 /* Present pass of one composite image (blit.fragment.glsl:32-35): crosshair
  * and vignette at texPos = ((x+0.5)/W, (y+0.5)/H); out RGBA32F and/or
  * display RGBA8 (clamp, round to nearest, R in the low byte), either NULL. */

@@ -35,6 +35,12 @@ class OrFrame(C.Structure):
                 ("display", C.c_void_p)]
 
 
+class OrPickResult(C.Structure):
+    _fields_ = [("data", C.c_uint32), ("face", C.c_uint32), ("steps", C.c_uint32),
+                ("_pad", C.c_uint32), ("hit_pos", C.c_float * 3), ("normal", C.c_float * 3),
+                ("place", C.c_int32 * 3), ("remove", C.c_int32 * 3)]
+
+
 class OrParams(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32),
                 ("max_steps_primary", C.c_int32), ("max_steps_shadow", C.c_int32),
@@ -67,6 +73,8 @@ def lib():
         L.or_sky.argtypes = [f3, f3, f3]
         L.or_render.argtypes = [C.POINTER(OrWorld), C.POINTER(OrCamera), C.POINTER(OrParams),
                                 C.POINTER(OrFrame), C.c_uint32, C.c_uint32, C.c_int]
+        L.or_pick.argtypes = [C.POINTER(OrWorld), C.POINTER(OrCamera), C.c_int,
+                              C.POINTER(OrPickResult)]
         L.or_present.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]
         L.or_shade_lambert.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, f3, C.c_float,
                                        C.c_void_p]
@@ -151,3 +159,10 @@ def present(composite):
     disp = np.zeros((H, W), np.uint32)
     lib().or_present(c.ctypes.data, W, H, out.ctypes.data, disp.ctypes.data)
     return out, disp
+
+
+def pick(world, cam, max_steps=64) -> OrPickResult:
+    r = OrPickResult()
+    oc = camera(cam)
+    lib().or_pick(C.byref(world.s), C.byref(oc), int(max_steps), C.byref(r))
+    return r

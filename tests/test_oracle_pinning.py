@@ -635,3 +635,59 @@ def test_present_rgba8_store():
     out2, disp2 = O.present(comp2)
     k = out2[0, 0, 0]
     assert disp2[0, 0] & 255 == int(np.floor(np.float32(k) * np.float32(255) + np.float32(0.5)))
+
+
+# ---------------------------------------------------------------------------
+# centre-ray pick (terrain_edit.comp.glsl:11-24)
+# ---------------------------------------------------------------------------
+
+def test_pick_is_the_centre_pixel_primary_ray(c0_render):
+    """rayUV = 0 is exactly the primary ray of pixel (W/2, H/2) of an even
+    image (uv = (W/2)/W*2-1 = 0), so the pick must report that pixel's hit
+    when traced with the pick's 64-step cap."""
+    grid, w, _, _, _, _ = c0_render
+    rng = np.random.default_rng(11)
+    W = H = 64
+    hits = 0
+    for k in range(24):
+        eye = (rng.uniform(0, 128), rng.uniform(70, 130), rng.uniform(-40, 0))
+        tgt = (rng.uniform(20, 108), rng.uniform(20, 60), rng.uniform(20, 108))
+        cam = vcam.look_at(eye, tgt, 70.0)
+        p = O.pick(w, cam, 64)
+        r = O.render(w, cam, W, H, 64, 64, S.SUN, rows=(H // 2, H // 2 + 1), gbuffer=True)
+        rec = r["hits"][H // 2, W // 2]
+        if rec[0] == 0xFFFFFFFF:
+            assert p.data == 0 and p.face == 0 and list(p.hit_pos) == [-1, -1, -1]
+            assert list(p.place) == [-1, -1, -1] and list(p.remove) == [-1, -1, -1]
+            continue
+        hits += 1
+        vid = int(rec[0])
+        block = (vid % 128, vid // 128 % 128, vid // 16384)
+        assert p.face == (rec[2] & 7) and p.data == S.palette()[grid[block[2], block[1], block[0]]]
+        assert tuple(p.remove) == block
+        n = np.array(p.normal, np.int32)
+        assert tuple(p.place) == tuple(np.array(block) + n)  # the empty neighbour cell
+        assert grid[p.place[2], p.place[1], p.place[0]] == 0 if min(p.place) >= 0 and max(p.place) < 128 else True
+        # the G-buffer position is the ceil of the same hit position / 8
+        pos = r["position"][H // 2, W // 2, :3]
+        assert np.array_equal(np.ceil(np.array(p.hit_pos, np.float32)) / np.float32(8), pos)
+    assert hits >= 12
+
+
+def test_pick_known_answer():
+    """A single voxel seen from above: face +y, remove = the voxel, place =
+    the cell on top of it; a camera looking away misses."""
+    grid = np.zeros((16, 16, 16), np.uint8)
+    grid[7, 3, 5] = 2  # (x, y, z) = (5, 3, 7)
+    pal = S.palette()
+    w = O.DenseWorld(grid, pal)
+    cam = vcam.look_at((5.5, 12.0, 7.4), (5.5, 0.0, 7.6), 70.0)
+    p = O.pick(w, cam, 64)
+    assert p.data == pal[2] and p.face == 4 and list(p.normal) == [0, 1, 0]
+    assert tuple(p.remove) == (5, 3, 7) and tuple(p.place) == (5, 4, 7)
+    # top face of block y=3 (sub-voxel y 32) as the reference leaves it after a
+    # negative crossing at the block level: grid 24 + within (1 << 3) * 0.999
+    assert p.hit_pos[1] == f32(24.0 + f32(8.0 * f32(0.999)))
+    away = vcam.look_at((5.5, 12.0, 7.4), (5.5, 30.0, 7.6), 70.0)
+    q = O.pick(w, away, 64)
+    assert q.data == 0 and tuple(q.place) == (-1, -1, -1)

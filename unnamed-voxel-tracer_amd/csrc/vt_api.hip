@@ -53,6 +53,7 @@ void free_world(vt_world* w) {
   dev_free(w->chunks);
   dev_free(w->pool);
   dev_free(w->atlas);
+  delete[] w->chunk_mirror;
   delete w;
 }
 
@@ -352,6 +353,13 @@ int vt_world_create_brickmap(vt_ctx* ctx, uint32_t dim, const uint32_t* chunks,
     return st;
   }
   w->pool_words = pool_words;
+  w->pool_capacity = pool_words ? pool_words : 512;
+  w->chunk_mirror = new (std::nothrow) uint32_t[nchunks];
+  if (!w->chunk_mirror) {
+    free_world(w);
+    return fail(VT_ERR_OUT_OF_MEMORY, "vt_world_create_brickmap: host chunk table");
+  }
+  std::memcpy(w->chunk_mirror, chunks, nchunks * 4);
   hipError_t e = hipMemcpy(w->chunks, chunks, nchunks * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess && pool_words)
     e = hipMemcpy(w->pool, pool, pool_words * 4, hipMemcpyHostToDevice);

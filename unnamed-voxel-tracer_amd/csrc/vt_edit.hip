@@ -1,0 +1,410 @@
+// vt_edit.hip -- world edits, voxel reads and the centre-ray pick.
+//
+// Edits replace VoxelBrickmap.set (src/engine/voxel.zig:46-61, the CPU
+// writing through a persistent mapping) and map_setVoxel (map.glsl:49-55);
+// reads replace VoxelBrickmap.get / map_getVoxel (voxel.zig:63-70,
+// map.glsl:38-47); the pick is terrain_edit.comp.glsl:11-17.
+//
+// A batch of edits means what calling set() once per edit, in order, means:
+// chunks are allocated on first touch in edit order (pool block index =
+// allocation count, voxel.zig:47-56), and of several edits of one voxel the
+// last wins.  The host resolves that order (chunk allocation, last-writer
+// dedup) and the device applies the surviving writes in one scatter:
+//   brickmap: pool words and new chunk-table entries; the pool grows by
+//             doubling like GpuBlockAllocator.alloc (gpu_block_allocator.zig:
+//             20-30), new blocks zero;
+//   dense:    Morton-ordered bytes, then the occupancy words of every touched
+//             4^3 brick (one wave per brick: ballot over its 64 bytes) and
+//             of every touched 16^3 region (ballot over its 64 brick words),
+//             then the compact albedo index, when built, is recomputed in
+//             place (its numbering changes with the occupancy).
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <unordered_map>
+#include <vector>
+
+#include "vt_internal.hpp"
+
+namespace vt {
+
+__global__ __launch_bounds__(256) void k_scatter_u32(uint32_t* __restrict__ dst,
+                                                     const uint64_t* __restrict__ addr,
+                                                     const uint32_t* __restrict__ val, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[addr[i]] = val[i];
+}
+
+__global__ __launch_bounds__(256) void k_scatter_u8(uint8_t* __restrict__ dst,
+                                                    const uint64_t* __restrict__ addr,
+                                                    const uint32_t* __restrict__ val, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[addr[i]] = (uint8_t)val[i];
+}
+
+// One wave per listed word: bit l of out[word] = (in[word * 64 + l] != 0).
+template <class T>
+__global__ __launch_bounds__(256) void k_rebuild_mask(const T* __restrict__ in,
+                                                      uint64_t* __restrict__ out,
+                                                      const uint32_t* __restrict__ words,
+                                                      uint32_t n) {
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t lane = threadIdx.x & 63u;
+  if (wave >= n) return;  // whole waves leave together
+  const uint32_t word = words[wave];
+  const uint64_t m = __ballot(in[(size_t)word * 64u + lane] != 0);
+  if (lane == 0) out[word] = m;
+}
+
+__global__ void k_gather_dense(const DenseWorld w, const uint32_t* __restrict__ xyz, uint32_t n,
+                               uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = w.vox[morton3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2])];
+}
+
+__global__ void k_gather_brickmap(const BrickmapWorld w, const uint32_t* __restrict__ xyz,
+                                  uint32_t n, uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = bm_block(w, (int)xyz[3 * i], (int)xyz[3 * i + 1], (int)xyz[3 * i + 2]);
+}
+
+// terrain_edit.comp.glsl:11-17: rayUV = 0, then the primary ray's
+// normalize(C_view * vec4(rayUV, 1, 1)).xyz (4 components), AABB entry and
+// EPSILON offset.
+__device__ __forceinline__ void centre_ray(const ViewConsts& v, uint32_t dim, float e[3],
+                                           float d[3]) {
+  float r[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    float s = v.view[i] * 0.0f;
+    s = s + v.view[4 + i] * 0.0f;
+    s = s + v.view[8 + i] * 1.0f;
+    s = s + v.view[12 + i] * 1.0f;
+    r[i] = s;
+  }
+  float q = r[0] * r[0];
+  q = q + r[1] * r[1];
+  q = q + r[2] * r[2];
+  q = q + r[3] * r[3];
+  const float len = sqrtf(q);
+  d[0] = r[0] / len;
+  d[1] = r[1] / len;
+  d[2] = r[2] / len;
+  float tn, tf;
+  const float D = (float)dim;
+  intersect_aabb(v.pos[0], v.pos[1], v.pos[2], d[0], d[1], d[2], 0.0f, 0.0f, 0.0f, D, D, D, tn,
+                 tf);
+  const float tc = fmaxf(tn, 0.0f);
+#pragma unroll
+  for (int i = 0; i < 3; i++) e[i] = (v.pos[i] + d[i] * tc) - 0.001f;
+}
+
+// HitInfo of the pick (map.glsl:127 on a hit, :167 on a miss) and the cells
+// of the shader's disabled edit body (terrain_edit.comp.glsl:21,23).
+__device__ __forceinline__ vt_pick_result pick_result(bool hit, uint32_t data, int face,
+                                                      float hx, float hy, float hz, int steps) {
+  vt_pick_result res = {};
+  res.steps = (uint32_t)steps;
+  if (hit) {
+    res.data = data;
+    res.face = (uint32_t)face;
+    res.hit_pos[0] = hx;
+    res.hit_pos[1] = hy;
+    res.hit_pos[2] = hz;
+    face_normal(face, res.normal[0], res.normal[1], res.normal[2]);
+  } else {
+    res.hit_pos[0] = res.hit_pos[1] = res.hit_pos[2] = -1.0f;
+  }
+  for (int i = 0; i < 3; i++) {
+    res.place[i] = f2i_sat(res.hit_pos[i] + res.normal[i]) >> 3;
+    res.remove[i] = f2i_sat(res.hit_pos[i]) >> 3;
+  }
+  return res;
+}
+
+__global__ void k_pick_dense(const DenseWorld w, ViewConsts v, int max_steps,
+                             vt_pick_result* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  float e[3], d[3];
+  centre_ray(v, w.dim, e, d);
+  DenseRay ray;
+  FlatLookup lk;
+  DenseHit h;
+  uint32_t l1f = 0, l0f = 0, leaf = 0;
+  if (!dense_begin<false>(ray, lk, w, e[0], e[1], e[2], d[0], d[1], d[2], max_steps, h, l1f, l0f))
+    dense_run<false>(ray, lk, w, h, l1f, l0f);
+  const uint32_t data = h.hit ? w.palette[lk.material(w, h.code, h.v, leaf)] : 0u;
+  *out = pick_result(h.hit, data, h.face, h.hx, h.hy, h.hz, h.steps);
+}
+
+__global__ void k_pick_brickmap(const BrickmapWorld w, ViewConsts v, int max_steps,
+                                vt_pick_result* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  float e[3], d[3];
+  centre_ray(v, w.dim, e, d);
+  GeneralHit g;
+  uint32_t leaf = 0;
+  trace_general<false>(w, e[0], e[1], e[2], d[0], d[1], d[2], max_steps, g, leaf);
+  *out = pick_result(g.data != 0u, g.data, g.face, g.hx, g.hy, g.hz, g.steps);
+}
+
+}  // namespace vt
+
+namespace {
+
+using vt::fail;
+using vt::hip_fail;
+
+// Device staging for one call; freed after the call's stream work is done.
+struct Staging {
+  std::vector<void*> bufs;
+  ~Staging() {
+    for (void* p : bufs) vt::dev_free(p);
+  }
+  template <class T>
+  hipError_t upload(const std::vector<T>& v, T** out, hipStream_t s) {
+    *out = nullptr;
+    if (v.empty()) return hipSuccess;
+    hipError_t e = hipMalloc((void**)out, v.size() * sizeof(T));
+    if (e != hipSuccess) return e;
+    bufs.push_back(*out);
+    return hipMemcpyAsync(*out, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+  }
+};
+
+uint32_t blocks_for(uint64_t n, uint32_t per_block) {
+  return (uint32_t)((n + per_block - 1) / per_block);
+}
+
+int check_coords(const vt_world* w, const uint32_t* xyz, uint32_t n, const char* who) {
+  for (uint32_t i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++)
+      if (xyz[3 * (size_t)i + k] >= w->dim)
+        return fail(VT_ERR_INVALID_ARGUMENT, "%s: voxel %u coordinate %u outside [0, %u)", who, i,
+                    xyz[3 * (size_t)i + k], w->dim);
+  return VT_OK;
+}
+
+// Grows the block pool to at least `blocks` blocks (doubling), keeping the
+// contents and zeroing the new blocks.
+int grow_pool(vt_world* w, uint64_t blocks, hipStream_t s) {
+  const uint64_t cap = w->pool_capacity / 512u;
+  if (blocks <= cap) return VT_OK;
+  uint64_t ncap = cap ? cap : 1;
+  while (ncap < blocks) ncap *= 2;
+  uint32_t* np = nullptr;
+  hipError_t e = hipMalloc((void**)&np, ncap * 512u * 4u);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return hip_fail(e, "vt_world_set_voxels: pool growth");
+  }
+  e = hipMemcpyAsync(np, w->pool, cap * 512u * 4u, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemsetAsync(np + cap * 512u, 0, (ncap - cap) * 512u * 4u, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    vt::dev_free(np);
+    return hip_fail(e, "vt_world_set_voxels: pool growth");
+  }
+  vt::dev_free(w->pool);
+  w->pool = np;
+  w->bytes += (ncap - cap) * 512u * 4u;
+  w->pool_capacity = ncap * 512u;
+  return VT_OK;
+}
+
+int set_brickmap(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint32_t n,
+                 hipStream_t s) {
+  const uint32_t M = w->dim / 8u;
+  std::vector<std::pair<uint64_t, uint32_t>> new_chunks;
+  std::unordered_map<uint64_t, uint32_t> last;  // pool word -> position in `writes`
+  std::vector<uint64_t> addr;
+  std::vector<uint32_t> val;
+  uint64_t used = w->pool_words / 512u;
+  last.reserve(n);
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t x = xyz[3 * (size_t)i], y = xyz[3 * (size_t)i + 1], z = xyz[3 * (size_t)i + 2];
+    const uint64_t ci = (uint64_t)(x / 8u) + M * ((uint64_t)(y / 8u) + (uint64_t)M * (z / 8u));
+    uint32_t& entry = w->chunk_mirror[ci];
+    if (entry == 0) {  // get_block_for_chunk: allocate on first touch
+      if (used >= 0xFFFFFFFEull) return fail(VT_ERR_OUT_OF_MEMORY, "vt_world_set_voxels: pool");
+      entry = (uint32_t)(++used);
+      new_chunks.emplace_back(ci, entry);
+    }
+    const uint64_t a = (uint64_t)(entry - 1u) * 512u + (x % 8u) + 8u * ((y % 8u) + 8u * (z % 8u));
+    auto it = last.find(a);
+    if (it == last.end()) {
+      last.emplace(a, (uint32_t)addr.size());
+      addr.push_back(a);
+      val.push_back(values[i]);
+    } else {
+      val[it->second] = values[i];
+    }
+  }
+  int st = grow_pool(w, used, s);
+  if (st != VT_OK) return st;  // (the mirror keeps the allocations: same as a
+                               //  reference resize panic, the world is unusable)
+  std::vector<uint64_t> caddr;
+  std::vector<uint32_t> cval;
+  for (auto& c : new_chunks) {
+    caddr.push_back(c.first);
+    cval.push_back(c.second);
+  }
+  Staging stg;
+  uint64_t *d_addr, *d_caddr;
+  uint32_t *d_val, *d_cval;
+  VT_HIP(stg.upload(addr, &d_addr, s));
+  VT_HIP(stg.upload(val, &d_val, s));
+  VT_HIP(stg.upload(caddr, &d_caddr, s));
+  VT_HIP(stg.upload(cval, &d_cval, s));
+  if (!addr.empty())
+    hipLaunchKernelGGL(vt::k_scatter_u32, dim3(blocks_for(addr.size(), 256)), dim3(256), 0, s,
+                       w->pool, d_addr, d_val, (uint32_t)addr.size());
+  if (!caddr.empty())
+    hipLaunchKernelGGL(vt::k_scatter_u32, dim3(blocks_for(caddr.size(), 256)), dim3(256), 0, s,
+                       w->chunks, d_caddr, d_cval, (uint32_t)caddr.size());
+  VT_HIP(hipGetLastError());
+  VT_HIP(hipStreamSynchronize(s));
+  w->pool_words = used * 512u;
+  return VT_OK;
+}
+
+int set_dense(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint32_t n,
+              hipStream_t s) {
+  for (uint32_t i = 0; i < n; i++)
+    if (values[i] > 255u)
+      return fail(VT_ERR_INVALID_ARGUMENT, "vt_world_set_voxels: dense value %u > 255 (edit %u)",
+                  values[i], i);
+  std::unordered_map<uint64_t, uint32_t> last;
+  std::vector<uint64_t> addr;
+  std::vector<uint32_t> val, bricks, regions;
+  last.reserve(n);
+  auto part = [](uint32_t v) {
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0xFF0000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    return (v | (v << 2)) & 0x09249249u;
+  };
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t code = part(xyz[3 * (size_t)i]) | (part(xyz[3 * (size_t)i + 1]) << 1) |
+                          (part(xyz[3 * (size_t)i + 2]) << 2);
+    auto it = last.find(code);
+    if (it == last.end()) {
+      last.emplace(code, (uint32_t)addr.size());
+      addr.push_back(code);
+      val.push_back(values[i]);
+      bricks.push_back(code >> 6);
+      regions.push_back(code >> 12);
+    } else {
+      val[it->second] = values[i];
+    }
+  }
+  std::sort(bricks.begin(), bricks.end());
+  bricks.erase(std::unique(bricks.begin(), bricks.end()), bricks.end());
+  std::sort(regions.begin(), regions.end());
+  regions.erase(std::unique(regions.begin(), regions.end()), regions.end());
+  Staging stg;
+  uint64_t* d_addr;
+  uint32_t *d_val, *d_bricks, *d_regions;
+  VT_HIP(stg.upload(addr, &d_addr, s));
+  VT_HIP(stg.upload(val, &d_val, s));
+  VT_HIP(stg.upload(bricks, &d_bricks, s));
+  VT_HIP(stg.upload(regions, &d_regions, s));
+  if (!addr.empty()) {
+    hipLaunchKernelGGL(vt::k_scatter_u8, dim3(blocks_for(addr.size(), 256)), dim3(256), 0, s,
+                       w->vox, d_addr, d_val, (uint32_t)addr.size());
+    hipLaunchKernelGGL((vt::k_rebuild_mask<uint8_t>), dim3(blocks_for(bricks.size(), 4)),
+                       dim3(256), 0, s, w->vox, w->l0, d_bricks, (uint32_t)bricks.size());
+    hipLaunchKernelGGL((vt::k_rebuild_mask<uint64_t>), dim3(blocks_for(regions.size(), 4)),
+                       dim3(256), 0, s, w->l0, w->l1, d_regions, (uint32_t)regions.size());
+  }
+  VT_HIP(hipGetLastError());
+  if (w->prefix && !addr.empty()) {  // keep the compact index current
+    const uint64_t nwords = (uint64_t)w->dim * w->dim * w->dim / 64u;
+    const uint64_t ntiles = (nwords + 1023) / 1024;
+    uint32_t* scratch = nullptr;
+    VT_HIP(hipMalloc((void**)&scratch, (ntiles + 1) * 4));
+    stg.bufs.push_back(scratch);
+    VT_HIP(vt::launch_compact_prefix(w->l0, nwords, w->prefix, scratch, s));
+    uint32_t total = 0;
+    VT_HIP(hipMemcpyAsync(&total, scratch + ntiles, 4, hipMemcpyDeviceToHost, s));
+    VT_HIP(hipStreamSynchronize(s));
+    w->n_occupied = total;
+  }
+  VT_HIP(hipStreamSynchronize(s));
+  return VT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vt_world_set_voxels(vt_ctx* ctx, vt_world* world, const uint32_t* xyz,
+                        const uint32_t* values, uint32_t n, void* stream) {
+  if (!ctx || !world || (n && (!xyz || !values)))
+    return fail(VT_ERR_NULL_POINTER, "vt_world_set_voxels: null argument");
+  if (world->ctx != ctx) return fail(VT_ERR_WRONG_CONTEXT, "vt_world_set_voxels");
+  int st = check_coords(world, xyz, n, "vt_world_set_voxels");
+  if (st != VT_OK || n == 0) return st;
+  VT_HIP(hipSetDevice(ctx->device));
+  const hipStream_t s = (hipStream_t)stream;
+  return world->kind == WorldKind::Dense ? set_dense(world, xyz, values, n, s)
+                                         : set_brickmap(world, xyz, values, n, s);
+}
+
+int vt_world_get_voxels(vt_ctx* ctx, const vt_world* world, const uint32_t* xyz, uint32_t n,
+                        uint32_t* values, void* stream) {
+  if (!ctx || !world || (n && (!xyz || !values)))
+    return fail(VT_ERR_NULL_POINTER, "vt_world_get_voxels: null argument");
+  if (world->ctx != ctx) return fail(VT_ERR_WRONG_CONTEXT, "vt_world_get_voxels");
+  int st = check_coords(world, xyz, n, "vt_world_get_voxels");
+  if (st != VT_OK || n == 0) return st;
+  VT_HIP(hipSetDevice(ctx->device));
+  const hipStream_t s = (hipStream_t)stream;
+  std::vector<uint32_t> coords(xyz, xyz + 3 * (size_t)n);
+  Staging stg;
+  uint32_t *d_xyz, *d_out;
+  VT_HIP(stg.upload(coords, &d_xyz, s));
+  VT_HIP(hipMalloc((void**)&d_out, (size_t)n * 4));
+  stg.bufs.push_back(d_out);
+  if (world->kind == WorldKind::Dense)
+    hipLaunchKernelGGL(vt::k_gather_dense, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                       vt::dense_view(world), d_xyz, n, d_out);
+  else
+    hipLaunchKernelGGL(vt::k_gather_brickmap, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                       vt::brickmap_view(world), d_xyz, n, d_out);
+  VT_HIP(hipGetLastError());
+  VT_HIP(hipMemcpyAsync(values, d_out, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+  VT_HIP(hipStreamSynchronize(s));
+  return VT_OK;
+}
+
+int vt_pick(vt_ctx* ctx, const vt_world* world, const vt_camera* camera, int32_t max_steps,
+            vt_pick_result* result, void* stream) {
+  if (!ctx || !world || !camera || !result)
+    return fail(VT_ERR_NULL_POINTER, "vt_pick: null argument");
+  if (world->ctx != ctx) return fail(VT_ERR_WRONG_CONTEXT, "vt_pick");
+  if (max_steps < 0 || max_steps > 65535)
+    return fail(VT_ERR_INVALID_ARGUMENT, "vt_pick: max_steps %d outside [0, 65535]", max_steps);
+  VT_HIP(hipSetDevice(ctx->device));
+  const hipStream_t s = (hipStream_t)stream;
+  const vt::ViewConsts v = vt::view_consts(camera, 1, 1);
+  vt_pick_result* d = nullptr;
+  VT_HIP(hipMalloc((void**)&d, sizeof *d));
+  if (world->kind == WorldKind::Dense)
+    hipLaunchKernelGGL(vt::k_pick_dense, dim3(1), dim3(64), 0, s, vt::dense_view(world), v,
+                       (int)max_steps, d);
+  else
+    hipLaunchKernelGGL(vt::k_pick_brickmap, dim3(1), dim3(64), 0, s, vt::brickmap_view(world), v,
+                       (int)max_steps, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(result, d, sizeof *d, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  vt::dev_free(d);
+  if (e != hipSuccess) return hip_fail(e, "vt_pick");
+  return VT_OK;
+}
+
+}  // extern "C"

@@ -31,7 +31,9 @@ struct vt_world {
   uint32_t* chunks;
   uint32_t* pool;
   uint32_t* atlas;
-  size_t pool_words;
+  size_t pool_words;     // words of allocated blocks (512 per block)
+  size_t pool_capacity;  // words the device pool holds
+  uint32_t* chunk_mirror;  // host copy of the chunk table (edit allocation)
   // compact albedo index of a dense world (built on request)
   uint32_t* prefix;     // occupied voxels before each L0 word
   uint64_t n_occupied;

@@ -77,12 +77,19 @@ class Tuning(C.Structure):
                 ("blocks_per_cu", C.c_uint32)]
 
 
+class PickResult(C.Structure):
+    _fields_ = [("data", C.c_uint32), ("face", C.c_uint32), ("steps", C.c_uint32),
+                ("_pad", C.c_uint32), ("hit_pos", C.c_float * 3), ("normal", C.c_float * 3),
+                ("place", C.c_int32 * 3), ("remove", C.c_int32 * 3)]
+
+
 VT_TILE_8X8, VT_TILE_16X4, VT_TILE_32X2 = 3, 4, 5
 
 assert C.sizeof(Camera) == 96
 assert C.sizeof(RenderParams) == 72
 assert C.sizeof(Frame) == 80
 assert C.sizeof(View) == 192
+assert C.sizeof(PickResult) == 64
 
 
 class VtError(RuntimeError):
@@ -120,6 +127,9 @@ def lib() -> C.CDLL:
         "vt_present": (i32, [vp, vp, u32, u32, vp, vp, vp]),
         "vt_vox_info": (i32, [vp, C.c_size_t, C.POINTER(u32), C.POINTER(u32)]),
         "vt_atlas_load_vox": (i32, [vp, C.c_size_t, vp, C.POINTER(u32)]),
+        "vt_world_set_voxels": (i32, [vp, vp, vp, vp, u32, vp]),
+        "vt_world_get_voxels": (i32, [vp, vp, vp, u32, vp, vp]),
+        "vt_pick": (i32, [vp, vp, C.POINTER(Camera), i32, C.POINTER(PickResult), vp]),
         "vt_world_build_compact_index": (i32, [vp, vp, C.POINTER(C.c_uint64)]),
         "vt_lambert_shade": (i32, [vp, vp, vp, u32, C.POINTER(RenderParams), vp, vp]),
         "vt_lambert_backward": (i32, [vp, vp, vp, u32, C.POINTER(RenderParams), vp, u32, vp, vp,

@@ -79,6 +79,36 @@ class World:
                 "vt_world_create_brickmap")
         return cls(ctx, h, dim, "brickmap")
 
+    @classmethod
+    def empty_brickmap(cls, ctx: Context, dim: int, atlas: np.ndarray) -> "World":
+        """VoxelBrickmap.init (voxel.zig:32-39): every chunk empty, no pool blocks."""
+        M = dim // 8
+        return cls.brickmap(ctx, dim, np.zeros(M ** 3, np.uint32), np.zeros(0, np.uint32), atlas)
+
+    def set_voxels(self, xyz, values, stream=None):
+        """Edits in order (last write of a voxel wins); see vt_world_set_voxels."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.uint32).reshape(-1, 3)
+        values = np.ascontiguousarray(values, dtype=np.uint32).reshape(-1)
+        assert len(values) == len(xyz)
+        N.check(N.lib().vt_world_set_voxels(self.ctx.h, self.h, xyz.ctypes.data,
+                                            values.ctypes.data, len(values),
+                                            C.c_void_p(_stream(stream))), "vt_world_set_voxels")
+
+    def get_voxels(self, xyz, stream=None) -> np.ndarray:
+        xyz = np.ascontiguousarray(xyz, dtype=np.uint32).reshape(-1, 3)
+        out = np.zeros(len(xyz), np.uint32)
+        N.check(N.lib().vt_world_get_voxels(self.ctx.h, self.h, xyz.ctypes.data, len(xyz),
+                                            out.ctypes.data, C.c_void_p(_stream(stream))),
+                "vt_world_get_voxels")
+        return out
+
+    def pick(self, cam: N.Camera, max_steps: int = 64, stream=None) -> N.PickResult:
+        """Centre-ray pick of terrain_edit.comp.glsl (64 steps there)."""
+        r = N.PickResult()
+        N.check(N.lib().vt_pick(self.ctx.h, self.h, C.byref(cam), max_steps, C.byref(r),
+                                C.c_void_p(_stream(stream))), "vt_pick")
+        return r
+
     def device_bytes(self) -> int:
         d, b = C.c_uint32(), C.c_size_t()
         N.check(N.lib().vt_world_info(self.h, C.byref(d), C.byref(b)), "vt_world_info")
@@ -254,6 +284,10 @@ def autotune(ctx: Context, world: World, views, prm: N.RenderParams, reps=3):
 
 def current_stream():
     return _torch().cuda.current_stream().cuda_stream
+
+
+def _stream(stream):
+    return current_stream() if stream is None else stream
 
 
 def render(ctx: Context, world: World, cam: N.Camera, frame: Frame, prm: N.RenderParams,
