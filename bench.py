@@ -418,6 +418,39 @@ def bench_diff(args, scene):
 
 AMBIENT_DIFF = 0.2
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def measured_traffic(key):
+    """HBM bytes per launch of a kernel/workload as measured with rocprofv3
+    FETCH_SIZE and WRITE_SIZE passes, scaled by the copy-kernel calibration
+    (profiles/traffic.json, written from the profile summaries); None if the
+    workload has not been measured."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        e = t["kernels"][key]
+        return int(e["fetch_bytes"] * t["fetch_scale"] + e["write_bytes"] * t["write_scale"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def bench_copy(args):
+    """Calibration for the HBM counters: one plain device-to-device copy of
+    1 GiB (torch's copy kernel) per step -- 1 GiB read + 1 GiB written."""
+    import torch
+    n = 1 << 28
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    stream = torch.cuda.current_stream()
+    ms = timed(torch, stream, lambda: dst.copy_(src), args.steps, args.warmup)
+    nbytes = 2 * n * 4
+    print(json.dumps({"metric": "copy GB/s (HBM counter calibration)",
+                      "value": round(nbytes / (ms * 1e-3) / 1e9, 2), "unit": "GB/s",
+                      "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": round(ms, 4), "higher_is_better": True,
+                      "bytes_read": n * 4, "bytes_written": n * 4}), flush=True)
+
 
 def main():
     ap = argparse.ArgumentParser()
@@ -425,11 +458,12 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--traversal", choices=["flat", "bitmask"], default="flat")
-    ap.add_argument("--workload", choices=["frame", "batched", "diff", "diff_batched"],
+    ap.add_argument("--workload", choices=["frame", "batched", "diff", "diff_batched", "copy"],
                     default="frame",
                     help="frame: 512^3, one 1080p view (the headline); batched: 1024^3, 64 "
                          "uneven cameras in one persistent graph launch; diff: 256^3, 16 "
-                         "views fwd+bwd; diff_batched: 1024^3 compact albedo, 48 views fwd+bwd")
+                         "views fwd+bwd; diff_batched: 1024^3 compact albedo, 48 views fwd+bwd; "
+                         "copy: 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -443,6 +477,8 @@ def main():
         return bench_diff(args, "c3")
     if args.workload == "diff_batched":
         return bench_diff(args, "c4")
+    if args.workload == "copy":
+        return bench_copy(args)
 
     import numpy as np
     import torch
@@ -528,7 +564,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": measured_traffic(f"k_render_dense/{args.traversal}/512cube_1080p"),
         },
         "cpu_baseline": cpu,
     }

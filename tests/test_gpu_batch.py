@@ -116,3 +116,70 @@ def test_autotune_picks_a_valid_tuning(ctx, setup):
     assert best.tile in (3, 4, 5) and best.chunk in (1, 2, 4, 8)
     assert best.traversal in (0, 1) and best.blocks_per_cu == 8
     assert 0.0 < ms < 1000.0
+
+
+def test_batch_present_matches_per_view(ctx, setup):
+    """VT_FLAG_PRESENT in a batch: each view presented with its own size."""
+    import torch
+    world, cams, _, _ = setup
+    flags = vt.VT_FLAG_PRESENT | vt.VT_FLAG_GBUF_UNORM8
+    ref, frames = [], []
+    for cam, (W, H) in zip(cams, SIZES):
+        f = vt.Frame.alloc(W, H, display=True)
+        vt.render(ctx, world, cam, f, vt.params(W, H, 512, 512, S.SUN, flags))
+        ref.append(f)
+        frames.append(vt.Frame.alloc(W, H, display=True))
+    b = vt.Batch(ctx, world, list(zip(cams, frames)), vt.params(0, 0, 512, 512, S.SUN, flags),
+                 vt.tuning(vt.VT_TILE_16X4, 2))
+    b.launch()
+    torch.cuda.synchronize()
+    print("[batch] present", flush=True)
+    for i, (f, r) in enumerate(zip(frames, ref)):
+        assert torch.equal(f.shaded, r.shaded) and torch.equal(f.display, r.display), i
+    b.close()
+
+
+@pytest.mark.parametrize("compact", [False, True])
+def test_batch_fused_gradient_sums_views(ctx, setup, compact):
+    """LAMBERT | GRAD_L2 in one batch launch: the loss and gradients are the
+    sums of the per-view fused renders (float atomics: rtol 1e-5)."""
+    import torch
+    world, cams, _, _ = setup
+    n = 128
+    flags = vt.VT_FLAG_LAMBERT | vt.VT_FLAG_GRAD_L2
+    nalb = n ** 3
+    if compact:
+        nalb = vt.build_compact_index(ctx, world)
+        flags |= vt.VT_FLAG_ALBEDO_COMPACT
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    albedo = torch.rand(nalb * 3, generator=g, device="cuda")
+    targets = [torch.rand((H, W, 4), generator=g, device="cuda") for (W, H) in SIZES]
+
+    def grads():
+        return (torch.zeros_like(albedo), torch.zeros(3, device="cuda"),
+                torch.zeros(1, device="cuda"))
+
+    ga, gs, gl = grads()
+    for cam, (W, H), t in zip(cams, SIZES, targets):
+        f = vt.Frame.alloc(W, H, hits=False)
+        f.target = t
+        vt.render(ctx, world, cam, f, vt.params(W, H, 512, 512, S.SUN, flags, 0, 0.2, albedo,
+                                                ga, gs, gl))
+    ba, bs, bl = grads()
+    frames = []
+    for (W, H), t in zip(SIZES, targets):
+        f = vt.Frame.alloc(W, H, hits=False)
+        f.target = t
+        frames.append(f)
+    b = vt.Batch(ctx, world, list(zip(cams, frames)),
+                 vt.params(0, 0, 512, 512, S.SUN, flags, 0, 0.2, albedo, ba, bs, bl),
+                 vt.tuning())
+    b.launch()
+    torch.cuda.synchronize()
+    print(f"[batch] fused gradient compact={compact}", flush=True)
+    assert torch.allclose(ba, ga, rtol=1e-5, atol=1e-5)
+    assert torch.allclose(bl, gl, rtol=1e-5)
+    assert torch.allclose(bs, gs, rtol=1e-4, atol=1e-2)
+    assert float(gl) > 0 and torch.count_nonzero(ga) > 1000
+    b.close()
