@@ -363,6 +363,8 @@ int vt_world_create_brickmap(vt_ctx* ctx, uint32_t dim, const uint32_t* chunks,
   hipError_t e = hipMemcpy(w->chunks, chunks, nchunks * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess && pool_words)
     e = hipMemcpy(w->pool, pool, pool_words * 4, hipMemcpyHostToDevice);
+  else if (e == hipSuccess)  // the spare block edits allocate first starts zero
+    e = hipMemset(w->pool, 0, 512 * 4);
   if (e == hipSuccess) e = hipMemcpy(w->atlas, atlas_rgba8, atlas_words * 4, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     free_world(w);
