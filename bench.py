@@ -349,13 +349,15 @@ def bench_diff(args, scene):
     print(f"[bench] loss stored {l_stored:.6g} recompute {l_rec:.6g} agree={agree} "
           f"gsun {s_stored.tolist()}", file=sys.stderr, flush=True)
 
-    # a few untimed descent steps: the gradients lower the loss
+    # a few untimed descent steps (Adam on the albedo): the gradients lower the loss
     losses = []
-    for _ in range(6):
+    opt = torch.optim.Adam([albedo], lr=0.05)
+    for _ in range(8):
         stored()
         torch.cuda.synchronize()
         losses.append(float(loss.item()))
-        albedo.sub_(0.02 * grad)
+        albedo.grad = grad
+        opt.step()
     torch.cuda.synchronize()
 
     world_b = world.device_bytes()

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256) void k_render_batch(DenseWorld w, ShadeConsts 
   const uint32_t tw = 1u << log2_tw, th = 64u >> log2_tw;
   const uint32_t home = blockIdx.x % kQueues;
   Counters cnt = {0, 0, 0, 0, 0, 0, 0, 0};
-  GradAcc gacc = {0.f, 0.f, 0.f, 0.f};
+  GradAcc gacc = grad_acc();
   for (uint32_t k = 0; k < kQueues; k++) {
     const uint32_t q = (home + k) % kQueues;
     const uint32_t lo = (uint32_t)(((uint64_t)total_tiles * q) / kQueues);
@@ -69,6 +69,7 @@ __global__ __launch_bounds__(256) void k_render_batch(DenseWorld w, ShadeConsts 
         const ViewConsts v = bv.v;
         if (px < v.W && py < v.H)
           trace_pixel_dense<L, kStats, kShade>(w, v, sc, bv.out, px, py, cnt, gacc);
+        if constexpr (kShade == kShadeLambertGrad) flush_albedo(sc, gacc);  // one pixel per lane
       }
     }
   }

@@ -4,8 +4,9 @@
 // A hit record (voxel id, t, face / shadow bit) fixes the discrete structure
 // of a pixel, so shading and gradients need no traversal: the backward reads
 // 12 B of record, 12 B of albedo and the upstream pixel, and adds 12 B of
-// gradient (float atomics: many pixels share a voxel) -- an HBM/atomic bound
-// pass.  dL/dsun is reduced per wave before its three atomics.
+// gradient.  Many pixels share a voxel: lanes hitting the same voxel in a row
+// are summed in registers first (one float-atomic triple per run), and
+// dL/dsun is reduced per wave before its three atomics.
 #include <cmath>
 
 #include "vt_pixel.hpp"
@@ -44,7 +45,7 @@ __global__ __launch_bounds__(256) void k_lambert_backward(DenseWorld w, ShadeCon
                                                           const float4* __restrict__ up,
                                                           uint32_t kind) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  GradAcc acc = {0.f, 0.f, 0.f, 0.f};
+  GradAcc acc = grad_acc();
   if (i < npix) {
     const uint32_t voxel = hits[3 * (size_t)i], info = hits[3 * (size_t)i + 2];
     const float4 u = up[i];
@@ -69,11 +70,12 @@ __global__ __launch_bounds__(256) void k_lambert_backward(DenseWorld w, ShadeCon
         g[1] = u.y;
         g[2] = u.z;
       }
-      lambert_accumulate(sc, ai, face, nd, occl, a, g, fct, ls, acc);
+      lambert_accumulate(ai, face, nd, occl, a, g, fct, ls, acc);
     } else if (kind == VT_UPSTREAM_L2_TARGET) {
       acc.loss += u.x * u.x + u.y * u.y + u.z * u.z;
     }
   }
+  flush_albedo(sc, acc);
   flush_grad(sc, acc);
 }
 

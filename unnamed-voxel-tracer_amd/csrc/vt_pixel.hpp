@@ -76,18 +76,30 @@ __device__ __forceinline__ uint32_t albedo_index(const DenseWorld& w, uint32_t f
 
 struct GradAcc {
   float loss, sx, sy, sz;  // per-lane partial sums, flushed per wave
+  uint32_t key;            // albedo index of the pending albedo gradient
+  float ga[3];             // pending dL/dalbedo of this lane's pixel
 };
 
-// Gradients of one hit pixel with dL/dI = g: dL/dA_c = g_c * fct, and, when
-// lit, dL/dsun = (sum_c g_c A_c) * (n - (n.l) l) / |sun|.
-__device__ __forceinline__ void lambert_accumulate(const ShadeConsts& sc, uint32_t aidx, int face,
-                                                   float nd, bool occluded, const float* A,
-                                                   const float g[3], float fct,
-                                                   const LambertSun& s, GradAcc& acc) {
-  float* ga = sc.grad_albedo + 3 * (size_t)aidx;
-  atomicAdd(ga + 0, g[0] * fct);
-  atomicAdd(ga + 1, g[1] * fct);
-  atomicAdd(ga + 2, g[2] * fct);
+constexpr uint32_t kNoKey = 0xFFFFFFFFu;
+
+__device__ __forceinline__ GradAcc grad_acc() {
+  GradAcc a;
+  a.loss = a.sx = a.sy = a.sz = 0.0f;
+  a.key = kNoKey;
+  a.ga[0] = a.ga[1] = a.ga[2] = 0.0f;
+  return a;
+}
+
+// Gradients of one hit pixel with dL/dI = g: dL/dA_c = g_c * fct (left
+// pending in acc until flush_albedo), and, when lit, dL/dsun =
+// (sum_c g_c A_c) * (n - (n.l) l) / |sun|.
+__device__ __forceinline__ void lambert_accumulate(uint32_t aidx, int face, float nd,
+                                                   bool occluded, const float* A, const float g[3],
+                                                   float fct, const LambertSun& s, GradAcc& acc) {
+  acc.key = aidx;
+  acc.ga[0] = g[0] * fct;
+  acc.ga[1] = g[1] * fct;
+  acc.ga[2] = g[2] * fct;
   if (nd > 0.0f && !occluded) {
     const float k = (g[0] * A[0] + g[1] * A[1] + g[2] * A[2]) / s.ls;
     float nx, ny, nz;
@@ -102,6 +114,44 @@ __device__ __forceinline__ float wave_sum_f(float x) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
   return x;
+}
+
+// Segmented sum over runs of equal keys in consecutive lanes (neighbouring
+// pixels of a row usually hit one voxel): on return the first lane of each
+// run holds the run's sums and gets true.  Every lane of the wave must call.
+__device__ __forceinline__ bool wave_run_sum(uint32_t key, float& a, float& b, float& c) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t prev = __shfl_up(key, 1, 64);
+  const bool head = lane == 0u || prev != key;
+  const uint64_t heads = __ballot(head);
+  const uint32_t run = (uint32_t)__popcll(heads & (~0ull >> (63u - lane)));
+#pragma unroll
+  for (uint32_t off = 1; off < 64u; off <<= 1) {
+    const float a2 = __shfl_down(a, off, 64), b2 = __shfl_down(b, off, 64),
+                c2 = __shfl_down(c, off, 64);
+    const uint32_t r2 = __shfl_down(run, off, 64);
+    if (lane + off < 64u && r2 == run) {
+      a += a2;
+      b += b2;
+      c += c2;
+    }
+  }
+  return head;
+}
+
+// Adds the wave's pending albedo gradients: one atomic triple per run of
+// lanes with the same albedo index.  Every lane of the wave must call this.
+__device__ __forceinline__ void flush_albedo(const ShadeConsts& sc, GradAcc& acc) {
+  float a = acc.ga[0], b = acc.ga[1], c = acc.ga[2];
+  const bool head = wave_run_sum(acc.key, a, b, c);
+  if (head && acc.key != kNoKey) {
+    float* g = sc.grad_albedo + 3 * (size_t)acc.key;
+    atomicAdd(g + 0, a);
+    atomicAdd(g + 1, b);
+    atomicAdd(g + 2, c);
+  }
+  acc.key = kNoKey;
+  acc.ga[0] = acc.ga[1] = acc.ga[2] = 0.0f;
 }
 
 // Every lane of the wave must call this.
@@ -224,7 +274,7 @@ __device__ __forceinline__ void write_pixel(const ViewConsts& v, const ShadeCons
         const float r[3] = {s.x - T.x, s.y - T.y, s.z - T.z};
         gacc.loss += r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
         const float g[3] = {2.0f * r[0], 2.0f * r[1], 2.0f * r[2]};
-        lambert_accumulate(sc, ai, p.face, nd, occl, a, g, fct, ls, gacc);
+        lambert_accumulate(ai, p.face, nd, occl, a, g, fct, ls, gacc);
       }
     } else if constexpr (grad) {
       gacc.loss += T.x * T.x + T.y * T.y + T.z * T.z;

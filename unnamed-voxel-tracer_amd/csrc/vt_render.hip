@@ -27,11 +27,14 @@ __global__ __launch_bounds__(256) void k_render_dense(DenseWorld w, ViewConsts v
   const uint32_t px = bx * (2u * tw) + (wave & 1u) * tw + (lane & (tw - 1u));
   const uint32_t py = by * (2u * th) + (wave >> 1) * th + (lane >> log2_tw);
   Counters cnt = {0, 0, 0, 0, 0, 0, 0, 0};
-  GradAcc gacc = {0.f, 0.f, 0.f, 0.f};
+  GradAcc gacc = grad_acc();
   if (px < v.W && py < v.H)
     trace_pixel_dense<L, kStats, kShade>(w, v, sc, out, px, py, cnt, gacc);
   if (kStats) flush_counters(out, cnt);
-  if constexpr (kShade == kShadeLambertGrad) flush_grad(sc, gacc);
+  if constexpr (kShade == kShadeLambertGrad) {
+    flush_albedo(sc, gacc);
+    flush_grad(sc, gacc);
+  }
 }
 
 template <bool kStats>
@@ -86,7 +89,7 @@ __global__ __launch_bounds__(256) void k_render_brickmap(BrickmapWorld w, ViewCo
       }
       ssteps = s.steps;
     }
-    GradAcc gacc = {0.f, 0.f, 0.f, 0.f};
+    GradAcc gacc = grad_acc();
     write_pixel<kShadeRef>(v, sc, out, px, py, p, shit, shx, shy, shz, sox, soy, soz, ssteps, cnt,
                            nullptr, gacc);
   }
