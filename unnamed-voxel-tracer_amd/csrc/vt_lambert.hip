@@ -7,6 +7,7 @@
 // gradient.  Many pixels share a voxel: lanes hitting the same voxel in a row
 // are summed in registers first (one float-atomic triple per run), and
 // dL/dsun is reduced per wave before its three atomics.
+#include <algorithm>
 #include <cmath>
 
 #include "vt_pixel.hpp"
@@ -44,38 +45,43 @@ __global__ __launch_bounds__(256) void k_lambert_backward(DenseWorld w, ShadeCon
                                                           uint32_t npix,
                                                           const float4* __restrict__ up,
                                                           uint32_t kind) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   GradAcc acc = grad_acc();
-  if (i < npix) {
-    const uint32_t voxel = hits[3 * (size_t)i], info = hits[3 * (size_t)i + 2];
-    const float4 u = up[i];
-    if (record_hit(w, voxel)) {
-      const int face = (int)(info & 7u);
-      const bool occl = ((info >> 3) & 1u) != 0u;
-      const LambertSun ls = lambert_sun(sc.sun);
-      const float nd = lambert_nd(face, ls);
-      const float fct = lambert_factor(sc.ambient, nd, occl);
-      const uint32_t code = (sc.flags & VT_FLAG_ALBEDO_COMPACT) ? voxel_code(w, voxel) : 0u;
-      const uint32_t ai = albedo_index(w, sc.flags, voxel, code);
-      const float* a = sc.albedo + 3 * (size_t)ai;
-      float g[3];
-      if (kind == VT_UPSTREAM_L2_TARGET) {
-        const float r[3] = {a[0] * fct - u.x, a[1] * fct - u.y, a[2] * fct - u.z};
-        acc.loss += r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-        g[0] = 2.0f * r[0];
-        g[1] = 2.0f * r[1];
-        g[2] = 2.0f * r[2];
-      } else {
-        g[0] = u.x;
-        g[1] = u.y;
-        g[2] = u.z;
+  // grid-stride over 256-pixel blocks (the bound is block-uniform, so every
+  // wave reaches each flush_albedo); the launch caps the grid so that the
+  // per-block loss / sun atomics stay few
+  for (uint32_t base = blockIdx.x * 256u; base < npix; base += gridDim.x * 256u) {
+    const uint32_t i = base + threadIdx.x;
+    if (i < npix) {
+      const uint32_t voxel = hits[3 * (size_t)i], info = hits[3 * (size_t)i + 2];
+      const float4 u = up[i];
+      if (record_hit(w, voxel)) {
+        const int face = (int)(info & 7u);
+        const bool occl = ((info >> 3) & 1u) != 0u;
+        const LambertSun ls = lambert_sun(sc.sun);
+        const float nd = lambert_nd(face, ls);
+        const float fct = lambert_factor(sc.ambient, nd, occl);
+        const uint32_t code = (sc.flags & VT_FLAG_ALBEDO_COMPACT) ? voxel_code(w, voxel) : 0u;
+        const uint32_t ai = albedo_index(w, sc.flags, voxel, code);
+        const float* a = sc.albedo + 3 * (size_t)ai;
+        float g[3];
+        if (kind == VT_UPSTREAM_L2_TARGET) {
+          const float r[3] = {a[0] * fct - u.x, a[1] * fct - u.y, a[2] * fct - u.z};
+          acc.loss += r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+          g[0] = 2.0f * r[0];
+          g[1] = 2.0f * r[1];
+          g[2] = 2.0f * r[2];
+        } else {
+          g[0] = u.x;
+          g[1] = u.y;
+          g[2] = u.z;
+        }
+        lambert_accumulate(ai, face, nd, occl, a, g, fct, ls, acc);
+      } else if (kind == VT_UPSTREAM_L2_TARGET) {
+        acc.loss += u.x * u.x + u.y * u.y + u.z * u.z;
       }
-      lambert_accumulate(ai, face, nd, occl, a, g, fct, ls, acc);
-    } else if (kind == VT_UPSTREAM_L2_TARGET) {
-      acc.loss += u.x * u.x + u.y * u.y + u.z * u.z;
     }
+    flush_albedo(sc, acc);
   }
-  flush_albedo(sc, acc);
   flush_grad(sc, acc);
 }
 
@@ -89,8 +95,9 @@ hipError_t launch_lambert_shade(const DenseWorld& w, const ShadeConsts& sc, cons
 hipError_t launch_lambert_backward(const DenseWorld& w, const ShadeConsts& sc,
                                    const uint32_t* hits, uint32_t npix, const float4* upstream,
                                    uint32_t kind, hipStream_t stream) {
-  hipLaunchKernelGGL(k_lambert_backward, dim3((npix + 255) / 256), dim3(256), 0, stream, w, sc,
-                     hits, npix, upstream, kind);
+  const uint32_t blocks = std::min<uint32_t>((npix + 255) / 256, 4096u);
+  hipLaunchKernelGGL(k_lambert_backward, dim3(blocks), dim3(256), 0, stream, w, sc, hits, npix,
+                     upstream, kind);
   return hipGetLastError();
 }
 

@@ -155,16 +155,25 @@ __device__ __forceinline__ void flush_albedo(const ShadeConsts& sc, GradAcc& acc
 }
 
 // Every lane of the wave must call this.
+// Loss and dL/dsun of the whole block: wave sums, then one sum over the
+// block's waves in LDS, then one atomic per quantity and block.  Float
+// atomics on one address serialise in the memory system, so they are kept
+// to a few thousand per launch.  Every thread of the block must call this.
 __device__ __forceinline__ void flush_grad(const ShadeConsts& sc, const GradAcc& acc) {
-  const float l = wave_sum_f(acc.loss), x = wave_sum_f(acc.sx), y = wave_sum_f(acc.sy),
-              z = wave_sum_f(acc.sz);
+  __shared__ float part[4][4];  // [quantity][wave], 256-thread blocks
+  const float v[4] = {wave_sum_f(acc.loss), wave_sum_f(acc.sx), wave_sum_f(acc.sy),
+                      wave_sum_f(acc.sz)};
+  const uint32_t wave = threadIdx.x >> 6;
   if ((threadIdx.x & 63u) == 0u) {
-    if (sc.loss) atomicAdd(sc.loss, l);
-    if (sc.grad_sun) {
-      atomicAdd(sc.grad_sun + 0, x);
-      atomicAdd(sc.grad_sun + 1, y);
-      atomicAdd(sc.grad_sun + 2, z);
-    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) part[q][wave] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < 4u) {
+    const uint32_t q = threadIdx.x;
+    const float t = (part[q][0] + part[q][1]) + (part[q][2] + part[q][3]);
+    float* dst = q == 0u ? sc.loss : (sc.grad_sun ? sc.grad_sun + (q - 1u) : nullptr);
+    if (dst) atomicAdd(dst, t);
   }
 }
 
