@@ -234,6 +234,7 @@ __device__ __forceinline__ bool in_dim(const DenseWorld& w, int bx, int by, int 
 // Occupancy probe strategies (identical answers by contract).
 struct FlatLookup {
   // one byte gather per probe; returns the material (0 = empty)
+  static constexpr bool kPrefetch = true;  // probes have no state: run ahead
   __device__ __forceinline__ void reset() {}
   template <bool kStats>
   __device__ __forceinline__ uint32_t probe(const DenseWorld& w, uint32_t code, uint32_t&,
@@ -250,6 +251,7 @@ struct BitmaskLookup {
   // keeps the last brick word in registers; a probe in another brick fetches
   // the region word when the 16^3 region changed, then the brick word only
   // when the region word says the brick is not empty.
+  static constexpr bool kPrefetch = false;
   uint32_t brick, region;
   uint64_t bword, rword;
   __device__ __forceinline__ void reset() {
@@ -405,9 +407,132 @@ __device__ __forceinline__ uint32_t morton_inc(uint32_t a, uint32_t M, bool& out
 // cross when two crossing times tie exactly (frequent for rays started on
 // lattice points, e.g. every shadow ray), so that correction is a short
 // branch inside the loop rather than an exit from it.
+// One block-level DDA step of traceMap (map.glsl:157-162) and the cell it
+// lands in; the state after the step is what the next lookup probes.
+struct DenseCell {
+  float wx, wy, wz;  // withinGridCoords
+  uint32_t c;        // flipped Morton code of gridsCoords >> 3
+  uint32_t M;        // mask of the axis crossed by the step (minIdx)
+  uint32_t a;        // Morton code of the probed block pos >> 3
+  bool wrapped;      // the step left the grid (traceMap's bounds test)
+  bool outside;      // the within >= 8 correction left the grid (empty)
+};
+
+// The step's ray constants are passed as scalars: the mask select below must
+// stay a select of values (a struct here was turned into a dynamically
+// indexed stack object, one scratch load per step).
+__device__ __forceinline__ DenseCell dense_step(float Px, float Py, float Pz, float Ax, float Ay,
+                                                float Az, float dx, float dy, float dz, float ix,
+                                                float iy, float iz, uint32_t mx, uint32_t my,
+                                                uint32_t mz, uint32_t flip, const DenseCell& s) {
+  DenseCell n;
+  const float tx = (Px - s.wx) * ix;
+  const float ty = (Py - s.wy) * iy;
+  const float tz = (Pz - s.wz) * iz;
+  const bool xy = tx < ty, xz = tx < tz, yz = ty < tz;
+  const bool m0 = xy & xz;
+  const bool m1 = (!xy) & yz;
+  const float tm = m0 ? tx : (m1 ? ty : tz);
+  const float nwx = s.wx + dx * tm;
+  const float nwy = s.wy + dy * tm;
+  const float nwz = s.wz + dz * tm;
+  n.wx = m0 ? Ax : nwx;
+  n.wy = m1 ? Ay : nwy;
+  n.wz = (m0 | m1) ? nwz : Az;
+  n.M = m0 ? mx : (m1 ? my : mz);
+  const uint32_t cm = s.c & n.M;
+  n.wrapped = cm == n.M;
+  n.c = ((cm - n.M) & n.M) | (s.c & ~n.M);
+  n.a = n.c ^ flip;
+  n.outside = false;
+  if (fmaxf(fmaxf(n.wx, n.wy), n.wz) >= 8.0f) {  // pos >> 3 = G + (int(within) >> 3)
+    if (n.wx >= 8.0f) n.a = morton_inc(n.a, mx, n.outside);
+    if (n.wy >= 8.0f) n.a = morton_inc(n.a, my, n.outside);
+    if (n.wz >= 8.0f) n.a = morton_inc(n.a, mz, n.outside);
+  }
+  return n;
+}
+
+// The flat lookup's loop, software-pipelined: the byte of step k+1's block is
+// requested before step k's byte is tested, so each gather's latency overlaps
+// one step of DDA arithmetic.  Every code a step produces (wrapped or
+// corrected) is a valid Morton code of the grid, so the run-ahead load is in
+// bounds; its result is discarded when the ray stops at step k.  Same steps,
+// same stopping rule and same final state as the plain loop.
+template <class L>
+__device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const DenseWorld& w,
+                                                   DenseHit& h) {
+  const float Px = r.Px, Py = r.Py, Pz = r.Pz, Ax = r.Ax, Ay = r.Ay, Az = r.Az;
+  const float dx = r.dx, dy = r.dy, dz = r.dz, ix = r.ix, iy = r.iy, iz = r.iz;
+  const uint32_t mx = w.mx, my = w.my, mz = w.mz, flip = r.flip;
+#define VT_STEP(s) dense_step(Px, Py, Pz, Ax, Ay, Az, dx, dy, dz, ix, iy, iz, mx, my, mz, flip, s)
+  const int max_steps = r.max_steps;
+  DenseCell s0;
+  s0.wx = r.wx;
+  s0.wy = r.wy;
+  s0.wz = r.wz;
+  s0.c = r.c;
+  DenseCell A = VT_STEP(s0), B;
+  int sA = r.steps + 1, sB = 0;
+  uint32_t vA = w.vox[A.a], vB = 0u;
+  bool useB;
+  for (;;) {
+    B = VT_STEP(A);
+    vB = w.vox[B.a];
+    if (A.outside) vA = 0u;
+    if (A.wrapped | (vA != 0u) | (sA >= max_steps)) {
+      useB = false;
+      break;
+    }
+    sB = sA + 1;
+    A = VT_STEP(B);
+    vA = w.vox[A.a];
+    if (B.outside) vB = 0u;
+    if (B.wrapped | (vB != 0u) | (sB >= max_steps)) {
+      useB = true;
+      break;
+    }
+    sA = sB + 1;
+  }
+#undef VT_STEP
+  (void)lk;
+  // the stopping state, field by field (a struct select would go through
+  // memory)
+  const int sF = useB ? sB : sA;
+  const bool wrappedF = useB ? B.wrapped : A.wrapped;
+  if (sF >= max_steps || wrappedF) {  // the loop cap, then the bounds test
+    h.hit = false;
+    h.steps = sF;
+    return;
+  }
+  const float fwx = useB ? B.wx : A.wx, fwy = useB ? B.wy : A.wy, fwz = useB ? B.wz : A.wz;
+  const uint32_t fc = useB ? B.c : A.c, fM = useB ? B.M : A.M, fa = useB ? B.a : A.a;
+  const uint32_t g = fc ^ flip;
+  // faceId = 2 minIdx + 2 - rayPositivity[minIdx] (map.glsl:119-125)
+  const bool fx = fM == mx, fy = fM == my;
+  const int pm = (flip & fM) ? 0 : 1;  // the complemented axes are the negative ones
+  h.hit = true;
+  h.face = (fx ? 2 : (fy ? 4 : 6)) - pm;
+  h.bx = (int)compact1by2(fa);
+  h.by = (int)compact1by2(fa >> 1);
+  h.bz = (int)compact1by2(fa >> 2);
+  h.code = fa;
+  h.v = useB ? vB : vA;
+  h.sub = (uint32_t)((int)fwx & 7) | ((uint32_t)((int)fwy & 7) << 3) |
+          ((uint32_t)((int)fwz & 7) << 6);
+  h.hx = (float)(compact1by2(g) << 3) + fwx;
+  h.hy = (float)(compact1by2(g >> 1) << 3) + fwy;
+  h.hz = (float)(compact1by2(g >> 2) << 3) + fwz;
+  h.steps = sF + 1;
+}
+
 template <bool kStats, class L>
 __device__ __forceinline__ void dense_run(DenseRay& r, L& lk, const DenseWorld& w, DenseHit& h,
                                           uint32_t& l1f, uint32_t& l0f) {
+  if constexpr (L::kPrefetch && !kStats) {
+    dense_run_prefetch(r, lk, w, h);
+    return;
+  }
   const float Px = r.Px, Py = r.Py, Pz = r.Pz, Ax = r.Ax, Ay = r.Ay, Az = r.Az;
   const float dx = r.dx, dy = r.dy, dz = r.dz, ix = r.ix, iy = r.iy, iz = r.iz;
   const uint32_t mx = w.mx, my = w.my, mz = w.mz, flip = r.flip;
@@ -454,9 +579,8 @@ __device__ __forceinline__ void dense_run(DenseRay& r, L& lk, const DenseWorld& 
     return;
   }
   const uint32_t g = c ^ flip;
-  const int m = M == mx ? 0 : (M == my ? 1 : 2);
   h.hit = true;
-  h.face = m == 0 ? 2 - r.px : (m == 1 ? 4 - r.py : 6 - r.pz);
+  h.face = (M == mx ? 2 : (M == my ? 4 : 6)) - ((flip & M) ? 0 : 1);
   h.bx = (int)compact1by2(a);
   h.by = (int)compact1by2(a >> 1);
   h.bz = (int)compact1by2(a >> 2);
