@@ -429,10 +429,15 @@ __device__ __forceinline__ DenseCell dense_step(float Px, float Py, float Pz, fl
   const float tx = (Px - s.wx) * ix;
   const float ty = (Py - s.wy) * iy;
   const float tz = (Pz - s.wz) * iz;
+  // minIdx = tx < ty ? (tx < tz ? 0 : 2) : (ty < tz ? 1 : 2): m1 = yz & !m0
+  // is the same rule.  t is never NaN (d != 0 after the 0.001 substitution,
+  // within is finite), so t[minIdx] is the smallest t; min3 can differ from
+  // it only in the sign of a zero, which no later operation can see
+  // (w + d * (+-0) = w, P - w, int(w) and hit_pos are unchanged).
   const bool xy = tx < ty, xz = tx < tz, yz = ty < tz;
   const bool m0 = xy & xz;
-  const bool m1 = (!xy) & yz;
-  const float tm = m0 ? tx : (m1 ? ty : tz);
+  const bool m1 = yz & !m0;
+  const float tm = fminf(fminf(tx, ty), tz);
   const float nwx = s.wx + dx * tm;
   const float nwy = s.wy + dy * tm;
   const float nwz = s.wz + dz * tm;
