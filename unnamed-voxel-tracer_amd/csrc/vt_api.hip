@@ -97,6 +97,7 @@ DenseWorld dense_view(const vt_world* w) {
   d.mx = 0x09249249u & low;
   d.my = (0x09249249u << 1) & low;
   d.mz = (0x09249249u << 2) & low;
+  d.nvox = w->dim * w->dim * w->dim;
   d.prefix = w->prefix;
   return d;
 }

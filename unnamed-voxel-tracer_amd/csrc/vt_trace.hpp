@@ -203,6 +203,7 @@ struct DenseWorld {
   uint32_t dim;             // voxels per axis (MAP_DIMENSION), a power of two
   uint32_t log2dim;
   uint32_t mx, my, mz;      // dilated masks of x, y, z over 3*log2(dim) bits
+  uint32_t nvox;            // dim^3 (bytes of vox)
   const uint32_t* prefix;   // compact albedo index: occupied voxels before each
                             // L0 word (null until built)
 };
@@ -413,10 +414,20 @@ struct DenseCell {
   float wx, wy, wz;  // withinGridCoords
   uint32_t c;        // flipped Morton code of gridsCoords >> 3
   uint32_t M;        // mask of the axis crossed by the step (minIdx)
-  uint32_t a;        // Morton code of the probed block pos >> 3
+  uint32_t a;        // Morton code of the probed block pos >> 3, or
+                     // kOutside when the within >= 8 correction left the grid
   bool wrapped;      // the step left the grid (traceMap's bounds test)
-  bool outside;      // the within >= 8 correction left the grid (empty)
 };
+
+// Beyond every grid (codes use at most 30 bits): a bounds-checked buffer load
+// of it returns 0, i.e. "empty", as map_getChunkFlags does outside the map.
+constexpr uint32_t kOutside = 0xFFFFFFFFu;
+
+// Real-space increment of one axis of a Morton code; kOutside when the
+// coordinate was already dim - 1 (kOutside stays kOutside).
+__device__ __forceinline__ uint32_t morton_inc_or_out(uint32_t a, uint32_t M) {
+  return (a & M) == M ? kOutside : ((((a | ~M) + 1u) & M) | (a & ~M));
+}
 
 // The step's ray constants are passed as scalars: the mask select below must
 // stay a select of values (a struct here was turned into a dynamically
@@ -449,20 +460,20 @@ __device__ __forceinline__ DenseCell dense_step(float Px, float Py, float Pz, fl
   n.wrapped = cm == n.M;
   n.c = ((cm - n.M) & n.M) | (s.c & ~n.M);
   n.a = n.c ^ flip;
-  n.outside = false;
   if (fmaxf(fmaxf(n.wx, n.wy), n.wz) >= 8.0f) {  // pos >> 3 = G + (int(within) >> 3)
-    if (n.wx >= 8.0f) n.a = morton_inc(n.a, mx, n.outside);
-    if (n.wy >= 8.0f) n.a = morton_inc(n.a, my, n.outside);
-    if (n.wz >= 8.0f) n.a = morton_inc(n.a, mz, n.outside);
+    if (n.wx >= 8.0f) n.a = morton_inc_or_out(n.a, mx);
+    if (n.wy >= 8.0f) n.a = morton_inc_or_out(n.a, my);
+    if (n.wz >= 8.0f) n.a = morton_inc_or_out(n.a, mz);
   }
   return n;
 }
 
 // The flat lookup's loop, software-pipelined: the byte of step k+1's block is
 // requested before step k's byte is tested, so each gather's latency overlaps
-// one step of DDA arithmetic.  Every code a step produces (wrapped or
-// corrected) is a valid Morton code of the grid, so the run-ahead load is in
-// bounds; its result is discarded when the ray stops at step k.  Same steps,
+// one step of DDA arithmetic; the run-ahead result is discarded when the ray
+// stops at step k.  The bytes are read with bounds-checked buffer loads
+// (base and size in a scalar resource, a 32-bit offset per lane): a wrapped
+// code is still a code of the grid, and kOutside reads as 0.  Same steps,
 // same stopping rule and same final state as the plain loop.
 template <class L>
 __device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const DenseWorld& w,
@@ -477,22 +488,23 @@ __device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const Den
   s0.wy = r.wy;
   s0.wz = r.wz;
   s0.c = r.c;
+  const __amdgpu_buffer_rsrc_t vox =
+      __builtin_amdgcn_make_buffer_rsrc((void*)w.vox, (short)0, (int)w.nvox, 0x00020000);
+#define VT_LOAD(a) (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(vox, (int)(a), 0, 0)
   DenseCell A = VT_STEP(s0), B;
   int sA = r.steps + 1, sB = 0;
-  uint32_t vA = w.vox[A.a], vB = 0u;
+  uint32_t vA = VT_LOAD(A.a), vB = 0u;
   bool useB;
   for (;;) {
     B = VT_STEP(A);
-    vB = w.vox[B.a];
-    if (A.outside) vA = 0u;
+    vB = VT_LOAD(B.a);
     if (A.wrapped | (vA != 0u) | (sA >= max_steps)) {
       useB = false;
       break;
     }
     sB = sA + 1;
     A = VT_STEP(B);
-    vA = w.vox[A.a];
-    if (B.outside) vB = 0u;
+    vA = VT_LOAD(A.a);
     if (B.wrapped | (vB != 0u) | (sB >= max_steps)) {
       useB = true;
       break;
@@ -500,6 +512,7 @@ __device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const Den
     sA = sB + 1;
   }
 #undef VT_STEP
+#undef VT_LOAD
   (void)lk;
   // the stopping state, field by field (a struct select would go through
   // memory)
