@@ -156,6 +156,80 @@ def bench_batched(args):
     ctx.close()
 
 
+def bench_brickmap(args):
+    """The drop-in path: the 512^3 headline world in the engine's own layout
+    (chunk table + block pool + 256^3 model atlas, one reference block per
+    grid voxel with a solid 8^3 atlas model), 1920x1080 primary + shadow, max
+    2048 steps, through the literal two-level traceMap kernel.  The hit
+    records must equal the dense world's."""
+    import torch
+
+    import vt
+    from vt import brickmap as BM
+    from vt import scenes as S
+
+    print("[bench] generating 512^3 world, reference layout", file=sys.stderr, flush=True)
+    grid = S.generate(S.spec("c1"))
+    pal = S.palette()
+    cam, W, H = S.views("c1")[0]
+    cap = S.MAX_STEPS["c1"]
+    ctx = vt.Context(0)
+    dim, chunks, pool, atlas = BM.from_dense(grid, pal)
+    world = vt.World.brickmap(ctx, dim, chunks, pool, atlas)
+    dense = vt.World.dense(ctx, grid, pal)
+    del grid
+    prm = vt.params(W, H, cap, cap, S.SUN)
+    fs = vt.Frame.alloc(W, H, stats=True)
+    vt.render(ctx, world, cam, fs, prm)
+    fd = vt.Frame.alloc(W, H)
+    vt.render(ctx, dense, cam, fd, prm)
+    torch.cuda.synchronize()
+    same = bool(torch.equal(fs.hits, fd.hits))
+    st = fs.stats_dict()
+    f = vt.Frame.alloc(W, H)
+    stream = torch.cuda.current_stream()
+    ms = timed(torch, stream, lambda: vt.render(ctx, world, cam, f, prm), args.steps, args.warmup)
+    dense_ms = timed(torch, stream, lambda: vt.render(ctx, dense, cam, fd, prm), args.steps,
+                     args.warmup)
+    npix = W * H
+    rays = npix + st["primary_hits"]
+    # 4 B chunk-table read per in-bounds step, 4 B pool read per step in an
+    # allocated chunk, 4 B atlas read per step in a nonempty block, + outputs
+    abytes = 4 * (st["l1_fetches"] + st["l0_fetches"]) + OUT_BYTES_PER_PIXEL * npix
+    achieved = abytes / (ms * 1e-3) / 1e9
+    line = {
+        "metric": "Mrays/s primary+shadow, 1080p/512³ grid in the engine's chunk/pool/atlas "
+                  "layout (drop-in path)",
+        "value": round(rays / (ms * 1e-3) / 1e6, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "512^3 world of the headline, as chunk table (64^3) + block pool "
+                        f"({len(pool) // 512} blocks) + 256^3 atlas; 1920x1080, max 2048 steps",
+            "hits_equal_dense": same,
+            "dense_ms": round(dense_ms, 4),
+            "chunk_reads": st["l1_fetches"],
+            "pool_atlas_reads": st["l0_fetches"],
+            "dda_steps": st["primary_steps"] + st["shadow_steps"],
+        },
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    world.close()
+    dense.close()
+    ctx.close()
+
+
 def _part1by2(t):
     t = t & 0x3FF
     t = (t | (t << 16)) & 0xFF0000FF
@@ -460,11 +534,13 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--traversal", choices=["flat", "bitmask"], default="flat")
-    ap.add_argument("--workload", choices=["frame", "batched", "diff", "diff_batched", "copy"],
+    ap.add_argument("--workload",
+                    choices=["frame", "batched", "diff", "diff_batched", "brickmap", "copy"],
                     default="frame",
                     help="frame: 512^3, one 1080p view (the headline); batched: 1024^3, 64 "
                          "uneven cameras in one persistent graph launch; diff: 256^3, 16 "
                          "views fwd+bwd; diff_batched: 1024^3 compact albedo, 48 views fwd+bwd; "
+                         "brickmap: the headline world in the engine's own layout; "
                          "copy: 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
@@ -481,6 +557,8 @@ def main():
         return bench_diff(args, "c4")
     if args.workload == "copy":
         return bench_copy(args)
+    if args.workload == "brickmap":
+        return bench_brickmap(args)
 
     import numpy as np
     import torch

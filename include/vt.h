@@ -128,8 +128,10 @@ enum {
   VT_STAT_SHADOW_STEPS = 1,  /* executed traceMap iterations, shadow rays   */
   VT_STAT_PRIMARY_HITS = 2,  /* = number of shadow rays traced              */
   VT_STAT_OCCLUDED = 3,      /* shadow rays that found an occluder          */
-  VT_STAT_L1_FETCHES = 4,    /* bitmask traversal: 16^3 region words loaded */
-  VT_STAT_L0_FETCHES = 5,    /* bitmask traversal: 4^3 brick words loaded   */
+  VT_STAT_L1_FETCHES = 4,    /* bitmask traversal: 16^3 region words loaded; */
+                             /* brickmap worlds: chunk-table reads          */
+  VT_STAT_L0_FETCHES = 5,    /* bitmask traversal: 4^3 brick words loaded;  */
+                             /* brickmap worlds: block-pool + atlas reads   */
   VT_STAT_LEAF_READS = 6,    /* bitmask traversal: material bytes read at   */
                              /* primary hits (flat: 0, its gathers are one  */
                              /* byte per executed step)                     */

@@ -145,8 +145,9 @@ __global__ void k_pick_brickmap(const BrickmapWorld w, ViewConsts v, int max_ste
   float e[3], d[3];
   centre_ray(v, w.dim, e, d);
   GeneralHit g;
-  uint32_t leaf = 0;
-  trace_general<false>(w, e[0], e[1], e[2], d[0], d[1], d[2], max_steps, g, leaf);
+  uint32_t chunk_reads = 0, pool_reads = 0;
+  trace_general<false>(w, e[0], e[1], e[2], d[0], d[1], d[2], max_steps, g, chunk_reads,
+                       pool_reads);
   *out = pick_result(g.data != 0u, g.data, g.face, g.hx, g.hy, g.hz, g.steps);
 }
 

@@ -622,14 +622,20 @@ struct BrickmapWorld {
   uint32_t dim;            // MAP_DIMENSION
 };
 
-// map_getVoxel (map.glsl:38-47) of block coordinates b
-__device__ __forceinline__ uint32_t bm_block(const BrickmapWorld& w, int bx, int by, int bz) {
+// map_getVoxel (map.glsl:38-47) of block coordinates b; counts the chunk-table
+// and pool reads it makes.
+template <bool kStats = false>
+__device__ __forceinline__ uint32_t bm_block(const BrickmapWorld& w, int bx, int by, int bz,
+                                             uint32_t* chunk_reads = nullptr,
+                                             uint32_t* pool_reads = nullptr) {
   int M = (int)(w.dim >> 3);
   int cx = bx >> 3, cy = by >> 3, cz = bz >> 3;
   if (cx < 0 || cy < 0 || cz < 0 || cx >= M || cy >= M || cz >= M) return 0;
   uint32_t idx = w.chunks[cx + M * (cy + cz * M)];
+  if (kStats) ++*chunk_reads;
   if (idx == 0) return 0;
   size_t o = (size_t)(idx - 1u) * 512u + (size_t)(bx % 8) + (size_t)(((bz % 8) * 8 + (by % 8)) * 8);
+  if (kStats) ++*pool_reads;
   return w.pool[o];
 }
 
@@ -658,7 +664,8 @@ struct GeneralHit {
 // traceMap (map.glsl:83-168) literally, both levels.
 template <bool kStats>
 __device__ bool trace_general(const BrickmapWorld& w, float ox, float oy, float oz, float dx,
-                              float dy, float dz, int max_steps, GeneralHit& h, uint32_t& leaf) {
+                              float dy, float dz, int max_steps, GeneralHit& h,
+                              uint32_t& chunk_reads, uint32_t& pool_atlas_reads) {
   if (dx == 0.0f) dx = 0.001f;
   if (dy == 0.0f) dy = 0.001f;
   if (dz == 0.0f) dz = 0.001f;
@@ -680,9 +687,10 @@ __device__ bool trace_general(const BrickmapWorld& w, float ox, float oy, float 
     if (g[0] >= B || g[1] >= B || g[2] >= B || g[0] < 0 || g[1] < 0 || g[2] < 0) break;
     int p[3] = {g[0] + (int)wc[0], g[1] + (int)wc[1], g[2] + (int)wc[2]};
     int b[3] = {p[0] >> 3, p[1] >> 3, p[2] >> 3};
-    uint32_t block = bm_block(w, b[0], b[1], b[2]);
+    uint32_t block = bm_block<kStats>(w, b[0], b[1], b[2], &chunk_reads, &pool_atlas_reads);
     if (block != 0u) {
       uint32_t sub = bm_sub(w, block, p[0] % 8, p[1] % 8, p[2] % 8);
+      if (kStats) pool_atlas_reads++;
       if (sub != 0u) {
         h.data = sub;
         h.face = m == 0 ? 2 - pv[0] : (m == 1 ? 4 - pv[1] : 6 - pv[2]);
