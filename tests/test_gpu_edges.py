@@ -1,0 +1,121 @@
+"""Edge cases of the render path on the GPU vs the oracle, through the C-ABI.
+
+Worlds: empty, completely solid, a checkerboard, single voxels in corners,
+random sparse grids, the smallest dense size (16^3); cameras: outside looking
+away (all sky), exactly on lattice points, inside a solid voxel, looking
+exactly along each axis (zero direction components: the 0.001
+substitution), grazing a face; images 1x1, 1xN, Nx1 and ragged tile edges;
+step caps 0, 1, 2 and a zero shadow cap; both traversals and the reference
+layout of the same worlds.  The bar is the one of the main parity tests:
+hit records, step counts, G-buffer position/normal/illumination and the
+statistics counters bit-exact, colours within 1e-5.
+"""
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+import vt
+from test_gpu_parity import compare, gpu_render
+from vt import brickmap as BM
+from vt import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def worlds():
+    rng = np.random.default_rng(21)
+    out = {}
+    out["empty16"] = np.zeros((16, 16, 16), np.uint8)
+    out["solid16"] = np.full((16, 16, 16), 3, np.uint8)
+    z, y, x = np.indices((32, 32, 32))
+    out["checker32"] = (((x + y + z) & 1) * (1 + (x % 8))).astype(np.uint8)
+    c = np.zeros((32, 32, 32), np.uint8)
+    for (a, b, d) in [(0, 0, 0), (31, 0, 0), (0, 31, 0), (0, 0, 31), (31, 31, 31), (15, 16, 15)]:
+        c[d, b, a] = 1 + (a + b + d) % 8
+    out["corners32"] = c
+    g = (rng.random((64, 64, 64)) < 0.02) * rng.integers(1, 9, (64, 64, 64))
+    out["sparse64"] = g.astype(np.uint8)
+    return out
+
+
+WORLDS = worlds()
+
+
+def cameras(n):
+    c = n / 2.0
+    return [
+        ("outside_away", vt.look_at((c, c, -3 * n), (c, c, -6 * n))),
+        ("lattice_point", vt.look_at((c, c, c), (c + 3.0, c - 2.0, c + 5.0))),
+        ("inside_voxel", vt.look_at((c + 0.3, c + 0.6, c + 0.45), (0.0, 0.0, 0.0))),
+        ("axis_pos_x", vt.look_at((-4.0, c + 0.5, c + 0.5), (n + 4.0, c + 0.5, c + 0.5))),
+        ("axis_neg_y", vt.look_at((c + 0.25, n + 6.0, c + 0.75), (c + 0.25, -6.0, c + 0.75),
+                                  up=(0.0, 0.0, 1.0))),
+        ("axis_neg_z", vt.look_at((c + 0.5, c + 0.5, n + 5.0), (c + 0.5, c + 0.5, -5.0))),
+        ("grazing_face", vt.look_at((-2.0, float(n // 2), -2.0), (n + 2.0, float(n // 2), n + 2.0))),
+        ("corner_view", vt.look_at((-1.0, -1.0, -1.0), (float(n), float(n), float(n)))),
+    ]
+
+
+SIZES = [(1, 1), (1, 7), (13, 1), (65, 3), (40, 33)]
+
+
+@pytest.mark.parametrize("wname", list(WORLDS))
+@pytest.mark.parametrize("traversal", [vt.VT_TRAVERSAL_FLAT, vt.VT_TRAVERSAL_BITMASK])
+def test_edge_worlds_cameras_sizes(ctx, wname, traversal):
+    grid = WORLDS[wname]
+    pal = S.palette()
+    n = grid.shape[0]
+    w = vt.World.dense(ctx, grid, pal)
+    ow = O.DenseWorld(grid, pal)
+    for cname, cam in cameras(n):
+        for (W, H) in SIZES:
+            label = f"{wname} {cname} {W}x{H} trav={traversal}"
+            g = gpu_render(ctx, w, cam, W, H, 4 * n, 4 * n, traversal=traversal)
+            o = O.render(ow, cam, W, H, 4 * n, 4 * n, S.SUN, traversal=traversal)
+            compare(g, o, label=label)
+    w.close()
+
+
+@pytest.mark.parametrize("caps", [(0, 0), (1, 0), (2, 1), (5, 0), (64, 2)])
+def test_edge_step_caps(ctx, caps):
+    grid = WORLDS["sparse64"]
+    pal = S.palette()
+    w = vt.World.dense(ctx, grid, pal)
+    ow = O.DenseWorld(grid, pal)
+    for cname, cam in cameras(64)[1:4]:
+        for traversal in (vt.VT_TRAVERSAL_FLAT, vt.VT_TRAVERSAL_BITMASK):
+            g = gpu_render(ctx, w, cam, 40, 33, caps[0], caps[1], traversal=traversal)
+            o = O.render(ow, cam, 40, 33, caps[0], caps[1], S.SUN, traversal=traversal)
+            compare(g, o, label=f"caps {caps} {cname} trav={traversal}")
+    w.close()
+
+
+@pytest.mark.parametrize("wname", ["solid16", "checker32", "corners32", "sparse64"])
+def test_edge_worlds_reference_layout(ctx, wname):
+    grid = WORLDS[wname]
+    pal = S.palette()
+    n = grid.shape[0]
+    dim, chunks, pool, atlas = BM.from_dense(grid, pal)
+    if pool.size == 0:
+        pytest.skip("empty world has no pool block")
+    w = vt.World.brickmap(ctx, dim, chunks, pool, atlas)
+    ow = O.BrickmapWorld(dim, chunks, pool, atlas)
+    for cname, cam in cameras(n):
+        g = gpu_render(ctx, w, cam, 40, 33, 8 * n, 8 * n)
+        o = O.render(ow, cam, 40, 33, 8 * n, 8 * n, S.SUN)
+        compare(g, o, stats=False, label=f"brickmap {wname} {cname}")
+    w.close()
+
+
+def test_edge_empty_reference_layout(ctx):
+    """An empty map (no pool block at all): every ray is sky."""
+    pal = S.palette()
+    dim, chunks, pool, atlas = BM.from_dense(WORLDS["empty16"], pal)
+    w = vt.World.brickmap(ctx, dim, chunks, pool, atlas)
+    ow = O.BrickmapWorld(dim, chunks, np.zeros(512, np.uint32), atlas)
+    cam = cameras(16)[1][1]
+    g = gpu_render(ctx, w, cam, 40, 33, 128, 128)
+    o = O.render(ow, cam, 40, 33, 128, 128, S.SUN)
+    compare(g, o, stats=False, label="empty reference layout")
+    assert np.all(g["hits"][..., 0] == 0xFFFFFFFF)
+    w.close()
