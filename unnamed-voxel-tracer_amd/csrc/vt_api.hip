@@ -53,6 +53,7 @@ void free_world(vt_world* w) {
   dev_free(w->chunks);
   dev_free(w->pool);
   dev_free(w->atlas);
+  dev_free(w->occ);
   delete[] w->chunk_mirror;
   delete w;
 }
@@ -108,6 +109,14 @@ BrickmapWorld brickmap_view(const vt_world* w) {
   b.pool = w->pool;
   b.atlas = w->atlas;
   b.dim = w->dim;
+  b.occ = w->occ;
+  b.nvox = w->dim * w->dim * w->dim;
+  uint32_t bits = 0;
+  while ((1u << bits) < w->dim) bits++;
+  const uint32_t low = bits * 3 >= 32 ? 0xFFFFFFFFu : ((1u << (bits * 3)) - 1u);
+  b.mx = 0x09249249u & low;
+  b.my = (0x09249249u << 1) & low;
+  b.mz = (0x09249249u << 2) & low;
   return b;
 }
 
@@ -370,6 +379,19 @@ int vt_world_create_brickmap(vt_ctx* ctx, uint32_t dim, const uint32_t* chunks,
   if (e != hipSuccess) {
     free_world(w);
     return hip_fail(e, "vt_world_create_brickmap: upload");
+  }
+  if ((dim & (dim - 1)) == 0) {  // power-of-two map: occupancy bytes for the fast loop
+    const size_t n = (size_t)dim * dim * dim;
+    if ((st = dev_alloc(w, (void**)&w->occ, n, "block occupancy")) != VT_OK) {
+      free_world(w);
+      return st;
+    }
+    e = vt::launch_build_occupancy(vt::brickmap_view(w), w->occ, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+      free_world(w);
+      return hip_fail(e, "vt_world_create_brickmap: occupancy");
+    }
   }
   *out = w;
   return VT_OK;

@@ -42,6 +42,14 @@ __global__ __launch_bounds__(256) void k_scatter_u8(uint8_t* __restrict__ dst,
   if (i < n) dst[addr[i]] = (uint8_t)val[i];
 }
 
+__global__ __launch_bounds__(256) void k_scatter_nonzero_u8(uint8_t* __restrict__ dst,
+                                                            const uint64_t* __restrict__ addr,
+                                                            const uint32_t* __restrict__ val,
+                                                            uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[addr[i]] = val[i] != 0u ? 1 : 0;
+}
+
 // One wave per listed word: bit l of out[word] = (in[word * 64 + l] != 0).
 template <class T>
 __global__ __launch_bounds__(256) void k_rebuild_mask(const T* __restrict__ in,
@@ -175,6 +183,18 @@ struct Staging {
   }
 };
 
+uint32_t part1by2_host(uint32_t v) {
+  v &= 0x3FFu;
+  v = (v | (v << 16)) & 0xFF0000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  return (v | (v << 2)) & 0x09249249u;
+}
+
+uint32_t morton_code(uint32_t x, uint32_t y, uint32_t z) {
+  return part1by2_host(x) | (part1by2_host(y) << 1) | (part1by2_host(z) << 2);
+}
+
 uint32_t blocks_for(uint64_t n, uint32_t per_block) {
   return (uint32_t)((n + per_block - 1) / per_block);
 }
@@ -220,7 +240,7 @@ int set_brickmap(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint3
   const uint32_t M = w->dim / 8u;
   std::vector<std::pair<uint64_t, uint32_t>> new_chunks;
   std::unordered_map<uint64_t, uint32_t> last;  // pool word -> position in `writes`
-  std::vector<uint64_t> addr;
+  std::vector<uint64_t> addr, oaddr;            // pool word, occupancy byte (Morton)
   std::vector<uint32_t> val;
   uint64_t used = w->pool_words / 512u;
   last.reserve(n);
@@ -239,6 +259,7 @@ int set_brickmap(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint3
       last.emplace(a, (uint32_t)addr.size());
       addr.push_back(a);
       val.push_back(values[i]);
+      if (w->occ) oaddr.push_back(morton_code(x, y, z));
     } else {
       val[it->second] = values[i];
     }
@@ -253,15 +274,19 @@ int set_brickmap(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint3
     cval.push_back(c.second);
   }
   Staging stg;
-  uint64_t *d_addr, *d_caddr;
+  uint64_t *d_addr, *d_caddr, *d_oaddr;
   uint32_t *d_val, *d_cval;
   VT_HIP(stg.upload(addr, &d_addr, s));
   VT_HIP(stg.upload(val, &d_val, s));
   VT_HIP(stg.upload(caddr, &d_caddr, s));
   VT_HIP(stg.upload(cval, &d_cval, s));
+  VT_HIP(stg.upload(oaddr, &d_oaddr, s));
   if (!addr.empty())
     hipLaunchKernelGGL(vt::k_scatter_u32, dim3(blocks_for(addr.size(), 256)), dim3(256), 0, s,
                        w->pool, d_addr, d_val, (uint32_t)addr.size());
+  if (!oaddr.empty())
+    hipLaunchKernelGGL(vt::k_scatter_nonzero_u8, dim3(blocks_for(oaddr.size(), 256)), dim3(256),
+                       0, s, w->occ, d_oaddr, d_val, (uint32_t)oaddr.size());
   if (!caddr.empty())
     hipLaunchKernelGGL(vt::k_scatter_u32, dim3(blocks_for(caddr.size(), 256)), dim3(256), 0, s,
                        w->chunks, d_caddr, d_cval, (uint32_t)caddr.size());
@@ -281,16 +306,9 @@ int set_dense(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint32_t
   std::vector<uint64_t> addr;
   std::vector<uint32_t> val, bricks, regions;
   last.reserve(n);
-  auto part = [](uint32_t v) {
-    v &= 0x3FFu;
-    v = (v | (v << 16)) & 0xFF0000FFu;
-    v = (v | (v << 8)) & 0x0300F00Fu;
-    v = (v | (v << 4)) & 0x030C30C3u;
-    return (v | (v << 2)) & 0x09249249u;
-  };
   for (uint32_t i = 0; i < n; i++) {
-    const uint32_t code = part(xyz[3 * (size_t)i]) | (part(xyz[3 * (size_t)i + 1]) << 1) |
-                          (part(xyz[3 * (size_t)i + 2]) << 2);
+    const uint32_t code =
+        morton_code(xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2]);
     auto it = last.find(code);
     if (it == last.end()) {
       last.emplace(code, (uint32_t)addr.size());

@@ -34,6 +34,7 @@ struct vt_world {
   size_t pool_words;     // words of allocated blocks (512 per block)
   size_t pool_capacity;  // words the device pool holds
   uint32_t* chunk_mirror;  // host copy of the chunk table (edit allocation)
+  uint8_t* occ;            // block occupancy in Morton order (power-of-two dim)
   // compact albedo index of a dense world (built on request)
   uint32_t* prefix;     // occupied voxels before each L0 word
   uint64_t n_occupied;
@@ -105,6 +106,8 @@ hipError_t launch_render_brickmap(const BrickmapWorld& w, const ViewConsts& v,
                                   hipStream_t stream);
 
 // World construction kernels.
+// occ[morton(b)] = (map_getVoxel(b) != 0) for every block b of a brickmap world.
+hipError_t launch_build_occupancy(const BrickmapWorld& w, uint8_t* occ, hipStream_t stream);
 hipError_t launch_build_dense(const uint8_t* linear, uint32_t dim, uint8_t* vox, uint64_t* l0,
                               uint64_t* l1, hipStream_t stream);
 // Exclusive prefix sum of popcount(l0[i]) into prefix[0..nwords); scratch

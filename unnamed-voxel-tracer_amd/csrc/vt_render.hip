@@ -60,7 +60,10 @@ __global__ __launch_bounds__(256) void k_render_brickmap(BrickmapWorld w, ViewCo
     float ey = (v.pos[1] + dy * tc) - 0.001f;
     float ez = (v.pos[2] + dz * tc) - 0.001f;
     GeneralHit g;
-    trace_general<kStats>(w, ex, ey, ez, dx, dy, dz, sc.max_primary, g, cnt.l1f, cnt.l0f);
+    if (!kStats && w.occ)
+      trace_brickmap_fast(w, ex, ey, ez, dx, dy, dz, sc.max_primary, g);
+    else
+      trace_general<kStats>(w, ex, ey, ez, dx, dy, dz, sc.max_primary, g, cnt.l1f, cnt.l0f);
     Primary p;
     p.hit = g.data != 0u;
     p.steps = g.steps;
@@ -79,8 +82,11 @@ __global__ __launch_bounds__(256) void k_render_brickmap(BrickmapWorld w, ViewCo
       p.hz = g.hz;
       shadow_origin(p, sc.flags, sox, soy, soz);
       GeneralHit s;
-      trace_general<kStats>(w, sox, soy, soz, sc.sun[0], sc.sun[1], sc.sun[2], sc.max_shadow, s,
-                            cnt.l1f, cnt.l0f);
+      if (!kStats && w.occ)
+        trace_brickmap_fast(w, sox, soy, soz, sc.sun[0], sc.sun[1], sc.sun[2], sc.max_shadow, s);
+      else
+        trace_general<kStats>(w, sox, soy, soz, sc.sun[0], sc.sun[1], sc.sun[2], sc.max_shadow,
+                              s, cnt.l1f, cnt.l0f);
       shit = s.data != 0u;
       if (shit) {
         shx = s.hx;

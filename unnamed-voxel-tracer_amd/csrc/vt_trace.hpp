@@ -475,24 +475,20 @@ __device__ __forceinline__ DenseCell dense_step(float Px, float Py, float Pz, fl
 // (base and size in a scalar resource, a 32-bit offset per lane): a wrapped
 // code is still a code of the grid, and kOutside reads as 0.  Same steps,
 // same stopping rule and same final state as the plain loop.
-template <class L>
-__device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const DenseWorld& w,
-                                                   DenseHit& h) {
-  const float Px = r.Px, Py = r.Py, Pz = r.Pz, Ax = r.Ax, Ay = r.Ay, Az = r.Az;
-  const float dx = r.dx, dy = r.dy, dz = r.dz, ix = r.ix, iy = r.iy, iz = r.iz;
-  const uint32_t mx = w.mx, my = w.my, mz = w.mz, flip = r.flip;
+// Block-level steps from s0 (whose own lookup was empty, `steps0` steps
+// done) over the byte array behind `bytes` until a step lands on a nonzero
+// byte, leaves the grid or reaches max_steps.  Returns the stopping state
+// field by field (a struct select would go through memory).
+__device__ __forceinline__ void dense_loop_prefetch(
+    float Px, float Py, float Pz, float Ax, float Ay, float Az, float dx, float dy, float dz,
+    float ix, float iy, float iz, uint32_t mx, uint32_t my, uint32_t mz, uint32_t flip,
+    const DenseCell& s0, int steps0, int max_steps, __amdgpu_buffer_rsrc_t bytes, float& fwx,
+    float& fwy, float& fwz, uint32_t& fc, uint32_t& fM, uint32_t& fa, uint32_t& fv, int& sF,
+    bool& wrappedF) {
 #define VT_STEP(s) dense_step(Px, Py, Pz, Ax, Ay, Az, dx, dy, dz, ix, iy, iz, mx, my, mz, flip, s)
-  const int max_steps = r.max_steps;
-  DenseCell s0;
-  s0.wx = r.wx;
-  s0.wy = r.wy;
-  s0.wz = r.wz;
-  s0.c = r.c;
-  const __amdgpu_buffer_rsrc_t vox =
-      __builtin_amdgcn_make_buffer_rsrc((void*)w.vox, (short)0, (int)w.nvox, 0x00020000);
-#define VT_LOAD(a) (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(vox, (int)(a), 0, 0)
+#define VT_LOAD(a) (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(bytes, (int)(a), 0, 0)
   DenseCell A = VT_STEP(s0), B;
-  int sA = r.steps + 1, sB = 0;
+  int sA = steps0 + 1, sB = 0;
   uint32_t vA = VT_LOAD(A.a), vB = 0u;
   bool useB;
   for (;;) {
@@ -513,18 +509,42 @@ __device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const Den
   }
 #undef VT_STEP
 #undef VT_LOAD
+  sF = useB ? sB : sA;
+  wrappedF = useB ? B.wrapped : A.wrapped;
+  fwx = useB ? B.wx : A.wx;
+  fwy = useB ? B.wy : A.wy;
+  fwz = useB ? B.wz : A.wz;
+  fc = useB ? B.c : A.c;
+  fM = useB ? B.M : A.M;
+  fa = useB ? B.a : A.a;
+  fv = useB ? vB : vA;
+}
+
+template <class L>
+__device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const DenseWorld& w,
+                                                   DenseHit& h) {
+  const uint32_t mx = w.mx, my = w.my, mz = w.mz, flip = r.flip;
+  const int max_steps = r.max_steps;
+  DenseCell s0;
+  s0.wx = r.wx;
+  s0.wy = r.wy;
+  s0.wz = r.wz;
+  s0.c = r.c;
+  const __amdgpu_buffer_rsrc_t vox =
+      __builtin_amdgcn_make_buffer_rsrc((void*)w.vox, (short)0, (int)w.nvox, 0x00020000);
+  float fwx, fwy, fwz;
+  uint32_t fc, fM, fa, fv;
+  int sF;
+  bool wrappedF;
+  dense_loop_prefetch(r.Px, r.Py, r.Pz, r.Ax, r.Ay, r.Az, r.dx, r.dy, r.dz, r.ix, r.iy, r.iz, mx,
+                      my, mz, flip, s0, r.steps, max_steps, vox, fwx, fwy, fwz, fc, fM, fa, fv,
+                      sF, wrappedF);
   (void)lk;
-  // the stopping state, field by field (a struct select would go through
-  // memory)
-  const int sF = useB ? sB : sA;
-  const bool wrappedF = useB ? B.wrapped : A.wrapped;
   if (sF >= max_steps || wrappedF) {  // the loop cap, then the bounds test
     h.hit = false;
     h.steps = sF;
     return;
   }
-  const float fwx = useB ? B.wx : A.wx, fwy = useB ? B.wy : A.wy, fwz = useB ? B.wz : A.wz;
-  const uint32_t fc = useB ? B.c : A.c, fM = useB ? B.M : A.M, fa = useB ? B.a : A.a;
   const uint32_t g = fc ^ flip;
   // faceId = 2 minIdx + 2 - rayPositivity[minIdx] (map.glsl:119-125)
   const bool fx = fM == mx, fy = fM == my;
@@ -535,7 +555,7 @@ __device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const Den
   h.by = (int)compact1by2(fa >> 1);
   h.bz = (int)compact1by2(fa >> 2);
   h.code = fa;
-  h.v = useB ? vB : vA;
+  h.v = fv;
   h.sub = (uint32_t)((int)fwx & 7) | ((uint32_t)((int)fwy & 7) << 3) |
           ((uint32_t)((int)fwz & 7) << 6);
   h.hx = (float)(compact1by2(g) << 3) + fwx;
@@ -620,6 +640,11 @@ struct BrickmapWorld {
   const uint32_t* pool;    // 512 u32 per block, in-block index x + 8y + 64z
   const uint32_t* atlas;   // 256^3 RGBA8 texels
   uint32_t dim;            // MAP_DIMENSION
+  // block occupancy for the empty-space loop (power-of-two dim only, else
+  // null): byte at morton(block) = (map_getVoxel(block) != 0)
+  const uint8_t* occ;
+  uint32_t nvox;           // dim^3
+  uint32_t mx, my, mz;     // dilated Morton masks of x, y, z
 };
 
 // map_getVoxel (map.glsl:38-47) of block coordinates b; counts the chunk-table
@@ -720,6 +745,128 @@ __device__ bool trace_general(const BrickmapWorld& w, float ox, float oy, float 
         g[i] -= lo;
       }
       s = 3;
+    }
+    float t[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) t[i] = ((float)(pv[i] << s) - wc[i]) * inv[i];
+    m = t[0] < t[1] ? (t[0] < t[2] ? 0 : 2) : (t[1] < t[2] ? 1 : 2);
+    float tm = m == 0 ? t[0] : (m == 1 ? t[1] : t[2]);
+    int sm = m == 0 ? sgn[0] : (m == 1 ? sgn[1] : sgn[2]);
+    int pm = m == 0 ? pv[0] : (m == 1 ? pv[1] : pv[2]);
+    float am = (float)((1 - pm) << s) * 0.999f;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      float nw = wc[i] + d[i] * tm;
+      wc[i] = (i == m) ? am : nw;
+      g[i] += (i == m) ? sm * (1 << s) : 0;
+    }
+  }
+  h.steps = step;
+  return false;
+}
+
+
+// traceMap over the reference layout with the empty-space steps on the fast
+// path: whenever the literal loop is at the block level (stepSize 3) and the
+// block it just looked up is empty, the following block-level steps run in
+// the pipelined Morton loop over the occupancy bytes (the same recurrence,
+// the same stopping rule) until a step lands on a nonempty block, leaves the
+// map or reaches the cap; the literal loop then resumes at that block's
+// lookup with the same grid / within / minIdx / step count.  Same hits, steps
+// and positions as trace_general.
+__device__ bool trace_brickmap_fast(const BrickmapWorld& w, float ox, float oy, float oz, float dx,
+                                    float dy, float dz, int max_steps, GeneralHit& h) {
+  if (dx == 0.0f) dx = 0.001f;
+  if (dy == 0.0f) dy = 0.001f;
+  if (dz == 0.0f) dz = 0.001f;
+  const int B = (int)(8u * w.dim);
+  int sgn[3] = {dx > 0.0f ? 1 : (dx < 0.0f ? -1 : 0), dy > 0.0f ? 1 : (dy < 0.0f ? -1 : 0),
+                dz > 0.0f ? 1 : (dz < 0.0f ? -1 : 0)};
+  int pv[3] = {(1 + sgn[0]) >> 1, (1 + sgn[1]) >> 1, (1 + sgn[2]) >> 1};
+  float d[3] = {dx, dy, dz};
+  float inv[3] = {1.0f / dx, 1.0f / dy, 1.0f / dz};
+  // block-level constants of the fast loop (stepSize 3)
+  const float P3[3] = {(float)(pv[0] << 3), (float)(pv[1] << 3), (float)(pv[2] << 3)};
+  const float A3[3] = {(float)((1 - pv[0]) << 3) * 0.999f, (float)((1 - pv[1]) << 3) * 0.999f,
+                       (float)((1 - pv[2]) << 3) * 0.999f};
+  const uint32_t flip = (pv[0] ? 0u : w.mx) | (pv[1] ? 0u : w.my) | (pv[2] ? 0u : w.mz);
+  const __amdgpu_buffer_rsrc_t occ =
+      __builtin_amdgcn_make_buffer_rsrc((void*)w.occ, (short)0, (int)w.nvox, 0x00020000);
+  int m = 0;
+  float o8[3] = {ox * 8.0f, oy * 8.0f, oz * 8.0f};
+  int g[3] = {f2i_sat(o8[0]), f2i_sat(o8[1]), f2i_sat(o8[2])};
+  float wc[3] = {o8[0] - (float)g[0], o8[1] - (float)g[1], o8[2] - (float)g[2]};
+  int s = 0;
+  h.data = 0;
+  h.hx = h.hy = h.hz = -1.0f;
+  int step;
+  for (step = 0; step < max_steps; step++) {
+    if (g[0] >= B || g[1] >= B || g[2] >= B || g[0] < 0 || g[1] < 0 || g[2] < 0) break;
+    int p[3] = {g[0] + (int)wc[0], g[1] + (int)wc[1], g[2] + (int)wc[2]};
+    int b[3] = {p[0] >> 3, p[1] >> 3, p[2] >> 3};
+    uint32_t block = bm_block(w, b[0], b[1], b[2]);
+    if (block != 0u) {
+      uint32_t sub = bm_sub(w, block, p[0] % 8, p[1] % 8, p[2] % 8);
+      if (sub != 0u) {
+        h.data = sub;
+        h.face = m == 0 ? 2 - pv[0] : (m == 1 ? 4 - pv[1] : 6 - pv[2]);
+        h.bx = b[0];
+        h.by = b[1];
+        h.bz = b[2];
+        h.sub = (uint32_t)(p[0] & 7) | ((uint32_t)(p[1] & 7) << 3) | ((uint32_t)(p[2] & 7) << 6);
+        h.hx = (float)g[0] + wc[0];
+        h.hy = (float)g[1] + wc[1];
+        h.hz = (float)g[2] + wc[2];
+        h.steps = step + 1;
+        return true;
+      }
+      if (s != 0) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          g[i] += (int)wc[i];
+          wc[i] = wc[i] - floorf(wc[i]);
+        }
+        s = 0;
+      }
+    } else {
+      if (s != 3) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          int lo = g[i] & 7;
+          wc[i] = wc[i] + (float)lo;
+          g[i] -= lo;
+        }
+        s = 3;
+      }
+      // empty block at the block level: the fast loop takes the next steps
+      DenseCell s0;
+      s0.wx = wc[0];
+      s0.wy = wc[1];
+      s0.wz = wc[2];
+      s0.c = morton3((uint32_t)g[0] >> 3, (uint32_t)g[1] >> 3, (uint32_t)g[2] >> 3) ^ flip;
+      float fw[3];
+      uint32_t fc, fM, fa, fv;
+      int sF;
+      bool wrappedF;
+      dense_loop_prefetch(P3[0], P3[1], P3[2], A3[0], A3[1], A3[2], d[0], d[1], d[2], inv[0],
+                          inv[1], inv[2], w.mx, w.my, w.mz, flip, s0, step, max_steps, occ, fw[0],
+                          fw[1], fw[2], fc, fM, fa, fv, sF, wrappedF);
+      (void)fa;
+      (void)fv;
+      if (wrappedF) {  // left the map: the bounds test of iteration sF
+        h.steps = sF;
+        return false;
+      }
+      const uint32_t gc = fc ^ flip;
+      g[0] = (int)(compact1by2(gc) << 3);
+      g[1] = (int)(compact1by2(gc >> 1) << 3);
+      g[2] = (int)(compact1by2(gc >> 2) << 3);
+      wc[0] = fw[0];
+      wc[1] = fw[1];
+      wc[2] = fw[2];
+      m = fM == w.mx ? 0 : (fM == w.my ? 1 : 2);
+      step = sF - 1;  // iteration sF starts with its bounds test and lookup
+      continue;
     }
     float t[3];
 #pragma unroll
