@@ -131,8 +131,9 @@ int main(void) {
   p.width = 0;
   st = vt_render(ctx, world, &cam, &p, &f, NULL);
   CHECK(st == VT_ERR_INVALID_ARGUMENT, "zero width -> %d", st);
-  st = vt_world_create_dense_u8(ctx, grid, 24, pal, &world);
-  CHECK(st == VT_ERR_INVALID_ARGUMENT, "dim 24 -> %d", st);
+  vt_world* bad = (vt_world*)&st;
+  st = vt_world_create_dense_u8(ctx, grid, 24, pal, &bad);
+  CHECK(st == VT_ERR_INVALID_ARGUMENT && bad == NULL, "dim 24 -> %d", st);
   CHECK(strlen(vt_status_string(VT_ERR_DEVICE)) > 0, "status string");
 
   CHECK(vt_world_destroy(world) == VT_OK, "vt_world_destroy");
