@@ -455,7 +455,7 @@ __device__ __forceinline__ DenseCell dense_step(float Px, float Py, float Pz, fl
   n.wx = m0 ? Ax : nwx;
   n.wy = m1 ? Ay : nwy;
   n.wz = (m0 | m1) ? nwz : Az;
-  n.M = m0 ? mx : (m1 ? my : mz);
+  n.M = mx << (m0 ? 0u : (yz ? 1u : 2u));  // my = mx << 1, mz = mx << 2
   const uint32_t cm = s.c & n.M;
   n.wrapped = cm == n.M;
   n.c = ((cm - n.M) & n.M) | (s.c & ~n.M);
@@ -520,6 +520,60 @@ __device__ __forceinline__ void dense_loop_prefetch(
   fv = useB ? vB : vA;
 }
 
+// Steps per iteration of the empty-space phase below.
+#ifndef VT_SKIP_STEPS
+#define VT_SKIP_STEPS 4
+#endif
+
+// Empty-space phase of the flat loop.  An iteration takes K block-level steps
+// from `s` (whose own lookup was empty, `steps` steps done) and requests all K
+// bytes without testing any of them in between; only when all K are zero and
+// no step left the grid does the state advance by K steps.  Otherwise `s` and
+// `steps` are left at the start of that iteration and the exact loop
+// (dense_loop_prefetch) replays it to the first stopping step.  The state
+// after K empty, in-grid steps is exactly the exact loop's state after those
+// steps, so the result is unchanged.  What this buys: one divergent exit per
+// K steps instead of one per step, and with it none of the per-step copies of
+// the exit state that a divergent exit costs.  An iteration only starts when
+// steps + K < max_steps, so the step cap falls inside the replay.
+template <int K>
+__device__ __forceinline__ void dense_skip_empty(float Px, float Py, float Pz, float Ax, float Ay,
+                                                 float Az, float dx, float dy, float dz,
+                                                 float ix, float iy, float iz, uint32_t mx,
+                                                 uint32_t my, uint32_t mz, uint32_t flip,
+                                                 DenseCell& s, int& steps, int max_steps,
+                                                 __amdgpu_buffer_rsrc_t bytes) {
+  float bx = s.wx, by = s.wy, bz = s.wz;
+  uint32_t bc = s.c;
+  int sb = steps;
+  while (sb + K < max_steps) {
+    DenseCell n;
+    n.wx = bx;
+    n.wy = by;
+    n.wz = bz;
+    n.c = bc;
+    uint32_t any = 0u;
+    bool wrapped = false;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      n = dense_step(Px, Py, Pz, Ax, Ay, Az, dx, dy, dz, ix, iy, iz, mx, my, mz, flip, n);
+      any |= (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(bytes, (int)n.a, 0, 0);
+      wrapped |= n.wrapped;
+    }
+    if (wrapped | (any != 0u)) break;
+    bx = n.wx;
+    by = n.wy;
+    bz = n.wz;
+    bc = n.c;
+    sb += K;
+  }
+  s.wx = bx;
+  s.wy = by;
+  s.wz = bz;
+  s.c = bc;
+  steps = sb;
+}
+
 template <class L>
 __device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const DenseWorld& w,
                                                    DenseHit& h) {
@@ -532,12 +586,15 @@ __device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const Den
   s0.c = r.c;
   const __amdgpu_buffer_rsrc_t vox =
       __builtin_amdgcn_make_buffer_rsrc((void*)w.vox, (short)0, (int)w.nvox, 0x00020000);
+  int steps0 = r.steps;
+  dense_skip_empty<VT_SKIP_STEPS>(r.Px, r.Py, r.Pz, r.Ax, r.Ay, r.Az, r.dx, r.dy, r.dz, r.ix, r.iy,
+                                  r.iz, mx, my, mz, flip, s0, steps0, max_steps, vox);
   float fwx, fwy, fwz;
   uint32_t fc, fM, fa, fv;
   int sF;
   bool wrappedF;
   dense_loop_prefetch(r.Px, r.Py, r.Pz, r.Ax, r.Ay, r.Az, r.dx, r.dy, r.dz, r.ix, r.iy, r.iz, mx,
-                      my, mz, flip, s0, r.steps, max_steps, vox, fwx, fwy, fwz, fc, fM, fa, fv,
+                      my, mz, flip, s0, steps0, max_steps, vox, fwx, fwy, fwz, fc, fM, fa, fv,
                       sF, wrappedF);
   (void)lk;
   if (sF >= max_steps || wrappedF) {  // the loop cap, then the bounds test
