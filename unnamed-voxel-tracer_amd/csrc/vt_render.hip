@@ -16,10 +16,12 @@
 
 namespace vt {
 
+// amdgpu_waves_per_eu(8): at most 64 VGPRs, so eight waves per SIMD hide the
+// gathers of the traversal loop (the few spills it costs lie outside it).
 template <class L, bool kStats, int kShade>
-__global__ __launch_bounds__(256) void k_render_dense(DenseWorld w, ViewConsts v, ShadeConsts sc,
-                                                      FrameOut out, uint32_t log2_tw,
-                                                      uint32_t blocks_x) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_render_dense(
+    DenseWorld w, ViewConsts v, ShadeConsts sc, FrameOut out, uint32_t log2_tw,
+    uint32_t blocks_x) {
   const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
   const uint32_t tw = 1u << log2_tw, th = 64u >> log2_tw;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;

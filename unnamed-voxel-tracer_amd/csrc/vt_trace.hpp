@@ -408,6 +408,17 @@ __device__ __forceinline__ uint32_t morton_inc(uint32_t a, uint32_t M, bool& out
 // cross when two crossing times tie exactly (frequent for rays started on
 // lattice points, e.g. every shadow ray), so that correction is a short
 // branch inside the loop rather than an exit from it.
+// A wave-uniform value held in a VGPR for the length of a loop.  The step's
+// mask select (two selects between the three axis masks) cannot read two
+// SGPRs in one instruction, and the compiler re-creates VGPR copies of the
+// masks with a v_mov at every step rather than keep three registers; an
+// opaque copy cannot be re-created, so it is made once and stays.
+__device__ __forceinline__ uint32_t in_vgpr(uint32_t s) {
+  uint32_t v;
+  asm("v_mov_b32 %0, %1" : "=v"(v) : "s"(s));
+  return v;
+}
+
 // One block-level DDA step of traceMap (map.glsl:157-162) and the cell it
 // lands in; the state after the step is what the next lookup probes.
 struct DenseCell {
@@ -455,7 +466,7 @@ __device__ __forceinline__ DenseCell dense_step(float Px, float Py, float Pz, fl
   n.wx = m0 ? Ax : nwx;
   n.wy = m1 ? Ay : nwy;
   n.wz = (m0 | m1) ? nwz : Az;
-  n.M = mx << (m0 ? 0u : (yz ? 1u : 2u));  // my = mx << 1, mz = mx << 2
+  n.M = m0 ? mx : (m1 ? my : mz);
   const uint32_t cm = s.c & n.M;
   n.wrapped = cm == n.M;
   n.c = ((cm - n.M) & n.M) | (s.c & ~n.M);
@@ -522,7 +533,7 @@ __device__ __forceinline__ void dense_loop_prefetch(
 
 // Steps per iteration of the empty-space phase below.
 #ifndef VT_SKIP_STEPS
-#define VT_SKIP_STEPS 4
+#define VT_SKIP_STEPS 5
 #endif
 
 // Empty-space phase of the flat loop.  An iteration takes K block-level steps
@@ -577,7 +588,7 @@ __device__ __forceinline__ void dense_skip_empty(float Px, float Py, float Pz, f
 template <class L>
 __device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const DenseWorld& w,
                                                    DenseHit& h) {
-  const uint32_t mx = w.mx, my = w.my, mz = w.mz, flip = r.flip;
+  const uint32_t mx = in_vgpr(w.mx), my = in_vgpr(w.my), mz = in_vgpr(w.mz), flip = r.flip;
   const int max_steps = r.max_steps;
   DenseCell s0;
   s0.wx = r.wx;
@@ -901,12 +912,17 @@ __device__ bool trace_brickmap_fast(const BrickmapWorld& w, float ox, float oy, 
       s0.wy = wc[1];
       s0.wz = wc[2];
       s0.c = morton3((uint32_t)g[0] >> 3, (uint32_t)g[1] >> 3, (uint32_t)g[2] >> 3) ^ flip;
+      const uint32_t mx = in_vgpr(w.mx), my = in_vgpr(w.my), mz = in_vgpr(w.mz);
+      int s0steps = step;
+      dense_skip_empty<VT_SKIP_STEPS>(P3[0], P3[1], P3[2], A3[0], A3[1], A3[2], d[0], d[1], d[2],
+                                      inv[0], inv[1], inv[2], mx, my, mz, flip, s0, s0steps,
+                                      max_steps, occ);
       float fw[3];
       uint32_t fc, fM, fa, fv;
       int sF;
       bool wrappedF;
       dense_loop_prefetch(P3[0], P3[1], P3[2], A3[0], A3[1], A3[2], d[0], d[1], d[2], inv[0],
-                          inv[1], inv[2], w.mx, w.my, w.mz, flip, s0, step, max_steps, occ, fw[0],
+                          inv[1], inv[2], mx, my, mz, flip, s0, s0steps, max_steps, occ, fw[0],
                           fw[1], fw[2], fc, fM, fa, fv, sF, wrappedF);
       (void)fa;
       (void)fv;
@@ -921,7 +937,7 @@ __device__ bool trace_brickmap_fast(const BrickmapWorld& w, float ox, float oy, 
       wc[0] = fw[0];
       wc[1] = fw[1];
       wc[2] = fw[2];
-      m = fM == w.mx ? 0 : (fM == w.my ? 1 : 2);
+      m = fM == mx ? 0 : (fM == my ? 1 : 2);
       step = sF - 1;  // iteration sF starts with its bounds test and lookup
       continue;
     }
