@@ -76,7 +76,7 @@ def test_edge_worlds_cameras_sizes(ctx, wname, traversal):
     w.close()
 
 
-@pytest.mark.parametrize("caps", [(0, 0), (1, 0), (2, 1), (5, 0), (64, 2)])
+@pytest.mark.parametrize("caps", [(0, 0), (1, 0), (2, 1), (5, 0), (6, 6), (10, 11), (64, 2)])
 def test_edge_step_caps(ctx, caps):
     grid = WORLDS["sparse64"]
     pal = S.palette()

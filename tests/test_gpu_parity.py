@@ -19,13 +19,7 @@ pytestmark = pytest.mark.gpu
 RGB_TOL = 1e-5
 
 
-def gpu_render(ctx, world, cam, W, H, max_p, max_s, flags=0, traversal=0, ambient=0.0,
-               albedo=None):
-    import torch
-    f = vt.Frame.alloc(W, H, gbuffer=True, steps=True, stats=True)
-    prm = vt.params(W, H, max_p, max_s, S.SUN, flags, traversal, ambient, albedo)
-    vt.render(ctx, world, cam, f, prm)
-    torch.cuda.synchronize()
+def _frame_dict(f):
     out = {
         "shaded": f.shaded.cpu().numpy(),
         "hits": f.hits.cpu().numpy().view(np.uint32),
@@ -34,12 +28,32 @@ def gpu_render(ctx, world, cam, W, H, max_p, max_s, flags=0, traversal=0, ambien
         "position": f.position.cpu().numpy(),
         "illumination": f.illumination.cpu().numpy(),
         "steps": f.steps.cpu().numpy().view(np.uint32),
-        "stats": f.stats.cpu().numpy().view(np.uint64),
     }
+    if f.stats is not None:
+        out["stats"] = f.stats.cpu().numpy().view(np.uint64)
+    return out
+
+
+def gpu_render(ctx, world, cam, W, H, max_p, max_s, flags=0, traversal=0, ambient=0.0,
+               albedo=None):
+    """Renders twice: with the statistics counters (the kernel variant that
+    counts every lookup) and without (the variant the benchmark times, with
+    its own traversal loops); compare() checks both against the oracle."""
+    import torch
+    prm = vt.params(W, H, max_p, max_s, S.SUN, flags, traversal, ambient, albedo)
+    f = vt.Frame.alloc(W, H, gbuffer=True, steps=True, stats=True)
+    vt.render(ctx, world, cam, f, prm)
+    ff = vt.Frame.alloc(W, H, gbuffer=True, steps=True)
+    vt.render(ctx, world, cam, ff, prm)
+    torch.cuda.synchronize()
+    out = _frame_dict(f)
+    out["fast"] = _frame_dict(ff)
     return out
 
 
 def compare(g, o, rows=None, stats=True, label=""):
+    if "fast" in g:
+        compare(g["fast"], o, rows=rows, stats=False, label=label + " [no-stats kernel]")
     sl = slice(None) if rows is None else slice(*rows)
     print(f"[parity] {label}", flush=True)
     gh, oh = g["hits"][sl], o["hits"][sl]
@@ -100,6 +114,7 @@ def test_c1_rows_and_flat_equals_bitmask(ctx):
     # full frame: flat and bitmask traversals return identical hits
     for k in ("hits", "steps", "position", "normal", "illumination"):
         assert np.array_equal(gf[k].view(np.uint32), gb[k].view(np.uint32)), k
+        assert np.array_equal(gf["fast"][k].view(np.uint32), gb["fast"][k].view(np.uint32)), k
     assert np.array_equal(gf["stats"][:4], gb["stats"][:4])
     # oracle on a band of rows through the horizon and the near terrain
     ow = O.DenseWorld(grid, pal)
@@ -126,6 +141,7 @@ def test_brickmap_embedding_a_matches_oracle_and_dense(ctx, c0):
     gd = gpu_render(ctx, wd, cam, W, H, 512, 512)
     for k in ("hits", "steps", "position", "normal", "illumination"):
         assert np.array_equal(g[k].view(np.uint32), gd[k].view(np.uint32)), k
+        assert np.array_equal(g["fast"][k].view(np.uint32), gd["fast"][k].view(np.uint32)), k
     w.close()
     wd.close()
 
