@@ -205,6 +205,69 @@ __host__ __device__ constexpr int shade_mode(uint32_t flags) {
                                    : ((flags & VT_FLAG_LAMBERT) ? kShadeLambert : kShadeRef);
 }
 
+// Colour image of a hit: unpackUnorm4x8(data) (primary.comp.glsl:58-60).
+__device__ __forceinline__ float4 hit_color(uint32_t data, bool unorm) {
+  float4 col = make_float4((float)(data & 255u) / 255.0f, (float)((data >> 8) & 255u) / 255.0f,
+                           (float)((data >> 16) & 255u) / 255.0f, (float)(data >> 24) / 255.0f);
+  if (unorm) col = make_float4(unorm8(col.x), unorm8(col.y), unorm8(col.z), unorm8(col.w));
+  return col;
+}
+
+// Normal and position images and the hit distance of a primary hit
+// (primary.comp.glsl:61-63).
+__device__ __forceinline__ void hit_normal_position(const ViewConsts& v, const Primary& p,
+                                                    bool unorm, float4& nrm, float4& pos,
+                                                    float& t) {
+  float nx, ny, nz;
+  face_normal(p.face, nx, ny, nz);
+  nrm = make_float4(nx, ny, nz, 1.0f);
+  if (unorm) nrm = make_float4(unorm8(nrm.x), unorm8(nrm.y), unorm8(nrm.z), unorm8(nrm.w));
+  pos = make_float4(ceilf(p.hx) / 8.0f, ceilf(p.hy) / 8.0f, ceilf(p.hz) / 8.0f, 1.0f);
+  t = len3(p.hx / 8.0f - v.pos[0], p.hy / 8.0f - v.pos[1], p.hz / 8.0f - v.pos[2]);
+}
+
+// traceEntities(rayOrigin, rayDir, distance(rayOrigin, inter.hit_pos / 8.))
+// and the illumination image of a hit (secondary.comp.glsl:36-50).
+__device__ __forceinline__ float4 shadow_illum(const ShadeConsts& sc, bool unorm, bool shit,
+                                               float shx, float shy, float shz, float sox,
+                                               float soy, float soz, bool& ent, bool& occl) {
+  float hx8 = shx / 8.0f, hy8 = shy / 8.0f, hz8 = shz / 8.0f;
+  float maxd = len3(sox - hx8, soy - hy8, soz - hz8);
+  ent = trace_entities(sox, soy, soz, sc.sun[0], sc.sun[1], sc.sun[2], maxd);
+  occl = ent || shit;
+  float4 ill = make_float4(sc.sun[0], sc.sun[1], sc.sun[2], occl ? -0.3f : 0.3f);
+  if (unorm) ill = make_float4(unorm8(ill.x), unorm8(ill.y), unorm8(ill.z), unorm8(ill.w));
+  return ill;
+}
+
+// Hit-record info word: face | shadow << 3 | sub-voxel << 4 | material << 16.
+__device__ __forceinline__ uint32_t hit_info(int face, uint32_t sub, uint32_t material,
+                                             bool occl) {
+  return (uint32_t)face | ((occl ? 1u : 0u) << 3) | (sub << 4) | (material << 16);
+}
+
+// Reference composite of a hit (blit.fragment.glsl:30-31), the optional
+// present pass, and its stores.
+__device__ __forceinline__ void store_ref_composite(const ViewConsts& v, const ShadeConsts& sc,
+                                                    const FrameOut& out, size_t pix, uint32_t px,
+                                                    uint32_t py, float4 col, float4 ill,
+                                                    bool hit) {
+  float4 s;
+  if (hit) {
+    float r, g, b;
+    sky(sc.sun, ill.x, ill.y, ill.z, r, g, b);
+    s = make_float4(col.x + ill.w * r, col.y + ill.w * g, col.z + ill.w * b,
+                    col.w + ill.w * 1.0f);
+  } else {
+    s = col;
+  }
+  if (sc.flags & VT_FLAG_PRESENT) {
+    s = present_pixel(s, px, py, v.W, v.H);
+    if (out.display) out.display[pix] = pack_unorm8(s);
+  }
+  out.shaded[pix] = s;
+}
+
 template <int kShade>
 __device__ __forceinline__ void write_pixel(const ViewConsts& v, const ShadeConsts& sc,
                                             const FrameOut& out, uint32_t px, uint32_t py,
@@ -218,24 +281,10 @@ __device__ __forceinline__ void write_pixel(const ViewConsts& v, const ShadeCons
   float t;
   bool occl = false;
   if (p.hit) {
-    col = make_float4((float)(p.data & 255u) / 255.0f, (float)((p.data >> 8) & 255u) / 255.0f,
-                      (float)((p.data >> 16) & 255u) / 255.0f, (float)(p.data >> 24) / 255.0f);
-    float nx, ny, nz;
-    face_normal(p.face, nx, ny, nz);
-    nrm = make_float4(nx, ny, nz, 1.0f);
-    pos = make_float4(ceilf(p.hx) / 8.0f, ceilf(p.hy) / 8.0f, ceilf(p.hz) / 8.0f, 1.0f);
-    if (unorm) {
-      col = make_float4(unorm8(col.x), unorm8(col.y), unorm8(col.z), unorm8(col.w));
-      nrm = make_float4(unorm8(nrm.x), unorm8(nrm.y), unorm8(nrm.z), unorm8(nrm.w));
-    }
-    // traceEntities(rayOrigin, rayDir, distance(rayOrigin, inter.hit_pos / 8.))
-    float hx8 = shx / 8.0f, hy8 = shy / 8.0f, hz8 = shz / 8.0f;
-    float maxd = len3(sox - hx8, soy - hy8, soz - hz8);
-    bool ent = trace_entities(sox, soy, soz, sc.sun[0], sc.sun[1], sc.sun[2], maxd);
-    occl = ent || shit;
-    ill = make_float4(sc.sun[0], sc.sun[1], sc.sun[2], occl ? -0.3f : 0.3f);
-    if (unorm) ill = make_float4(unorm8(ill.x), unorm8(ill.y), unorm8(ill.z), unorm8(ill.w));
-    t = len3(p.hx / 8.0f - v.pos[0], p.hy / 8.0f - v.pos[1], p.hz / 8.0f - v.pos[2]);
+    col = hit_color(p.data, unorm);
+    hit_normal_position(v, p, unorm, nrm, pos, t);
+    bool ent;
+    ill = shadow_illum(sc, unorm, shit, shx, shy, shz, sox, soy, soz, ent, occl);
     cnt.hits++;
     cnt.occluded += occl ? 1u : 0u;
     cnt.ent += ent ? 1u : 0u;
@@ -260,9 +309,7 @@ __device__ __forceinline__ void write_pixel(const ViewConsts& v, const ShadeCons
   if (out.illum) out.illum[pix] = ill;
   if (out.steps) out.steps[pix] = (uint32_t)p.steps | ((uint32_t)(p.hit ? ssteps : 0) << 16);
   if (out.hits) {
-    uint32_t info = p.hit ? ((uint32_t)p.face | ((occl ? 1u : 0u) << 3) | (p.sub << 4) |
-                             (p.material << 16))
-                          : 0u;
+    uint32_t info = p.hit ? hit_info(p.face, p.sub, p.material, occl) : 0u;
     out.hits[3 * pix + 0] = p.hit ? p.voxel : 0xFFFFFFFFu;
     out.hits[3 * pix + 1] = __float_as_uint(t);
     out.hits[3 * pix + 2] = info;
@@ -290,21 +337,45 @@ __device__ __forceinline__ void write_pixel(const ViewConsts& v, const ShadeCons
     }
     if (out.shaded) out.shaded[pix] = s;
   } else if (out.shaded) {
-    float4 s;
-    if (p.hit) {
-      float r, g, b;
-      sky(sc.sun, ill.x, ill.y, ill.z, r, g, b);
-      s = make_float4(col.x + ill.w * r, col.y + ill.w * g, col.z + ill.w * b,
-                      col.w + ill.w * 1.0f);
-    } else {
-      s = col;
-    }
-    if (sc.flags & VT_FLAG_PRESENT) {
-      s = present_pixel(s, px, py, v.W, v.H);
-      if (out.display) out.display[pix] = pack_unorm8(s);
-    }
-    out.shaded[pix] = s;
+    store_ref_composite(v, sc, out, pix, px, py, col, ill, p.hit);
   }
+}
+
+// Stores of a hit pixel whose shadow ray was deferred: the ones that do not
+// depend on it (colour, normal, position, hit-record voxel and distance).
+__device__ __forceinline__ void write_pixel_deferred(const ViewConsts& v, const ShadeConsts& sc,
+                                                     const FrameOut& out, size_t pix,
+                                                     const Primary& p) {
+  const bool unorm = (sc.flags & VT_FLAG_GBUF_UNORM8) != 0u;
+  float4 nrm, pos;
+  float t;
+  hit_normal_position(v, p, unorm, nrm, pos, t);
+  if (out.color) out.color[pix] = hit_color(p.data, unorm);
+  if (out.normal) out.normal[pix] = nrm;
+  if (out.position) out.position[pix] = pos;
+  if (out.hits) {
+    out.hits[3 * pix + 0] = p.voxel;
+    out.hits[3 * pix + 1] = __float_as_uint(t);
+  }
+}
+
+// The remaining stores of a deferred pixel once its shadow ray is finished
+// (h): illumination, step counts, hit-record info word, composite.
+__device__ __forceinline__ void write_pixel_shadow(const ViewConsts& v, const ShadeConsts& sc,
+                                                   const FrameOut& out, const ShadowJob& j,
+                                                   const DenseHit& h) {
+  const size_t pix = j.pix;
+  const bool unorm = (sc.flags & VT_FLAG_GBUF_UNORM8) != 0u;
+  const float shx = h.hit ? h.hx : -1.0f, shy = h.hit ? h.hy : -1.0f,
+              shz = h.hit ? h.hz : -1.0f;
+  bool ent, occl;
+  const float4 ill = shadow_illum(sc, unorm, h.hit, shx, shy, shz, j.sox, j.soy, j.soz, ent, occl);
+  if (out.illum) out.illum[pix] = ill;
+  if (out.steps) out.steps[pix] = j.psteps | ((uint32_t)h.steps << 16);
+  if (out.hits) out.hits[3 * pix + 2] = j.info | ((occl ? 1u : 0u) << 3);
+  if (out.shaded)
+    store_ref_composite(v, sc, out, pix, j.pix % v.W, j.pix / v.W, hit_color(j.data, unorm), ill,
+                        true);
 }
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
@@ -322,13 +393,34 @@ __device__ __forceinline__ void flush_counters(const FrameOut& out, const Counte
   }
 }
 
+// Appends the deferred shadow rays of a wave to the job list: one atomic per
+// wave, slots in lane order.  Every lane of the wave that traced a pixel must
+// call this.
+__device__ __forceinline__ void enqueue_shadow_job(const ShadowQueue& q, bool defer,
+                                                   const ShadowJob& job) {
+  const unsigned long long m = __ballot(defer);
+  if (m == 0ull) return;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t leader = (uint32_t)__ffsll(m) - 1u;
+  uint32_t base = 0u;
+  if (lane == leader) base = atomicAdd(q.count, (uint32_t)__popcll(m));
+  base = __shfl(base, (int)leader, 64);
+  if (defer) q.jobs[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = job;
+}
+
 // One pixel of a dense world: primary ray (primary.comp.glsl:31-68), then the
-// shadow ray of a hit (secondary.comp.glsl:25-50), then every store.
-template <class L, bool kStats, int kShade>
+// shadow ray of a hit (secondary.comp.glsl:25-50), then every store.  With
+// kDefer (flat lookup, reference composite, no statistics) a shadow ray still
+// running after kShadowPauseSteps steps goes to the job list `q` instead: the
+// pixel's shadow-independent stores are made here, the rest by the shadow
+// pass (k_shadow_finish) once the ray is finished.
+template <class L, bool kStats, int kShade, bool kDefer = false>
 __device__ __forceinline__ void trace_pixel_dense(const DenseWorld& w, const ViewConsts& v,
                                                   const ShadeConsts& sc, const FrameOut& out,
                                                   uint32_t px, uint32_t py, Counters& cnt,
-                                                  GradAcc& gacc) {
+                                                  GradAcc& gacc, const ShadowQueue* q = nullptr) {
+  static_assert(!kDefer || (L::kPrefetch && !kStats && kShade == kShadeRef),
+                "shadow deferral: flat lookup, reference composite, no statistics");
   DenseRay r;
   L lk;
   DenseHit h;
@@ -352,6 +444,7 @@ __device__ __forceinline__ void trace_pixel_dense(const DenseWorld& w, const Vie
   bool shit = false;
   float shx = -1.0f, shy = -1.0f, shz = -1.0f;
   int ssteps = 0;
+  bool deferred = false;
   if (p.hit) {
     p.face = h.face;
     p.voxel = (uint32_t)h.bx + w.dim * ((uint32_t)h.by + w.dim * (uint32_t)h.bz);
@@ -365,8 +458,12 @@ __device__ __forceinline__ void trace_pixel_dense(const DenseWorld& w, const Vie
     // ---- shadow ray (secondary.comp.glsl:25-42) ----
     shadow_origin(p, sc.flags, sox, soy, soz);
     if (!dense_begin<kStats>(r, lk, w, sox, soy, soz, sc.sun[0], sc.sun[1], sc.sun[2],
-                             sc.max_shadow, h, cnt.l1f, cnt.l0f))
-      dense_run<kStats>(r, lk, w, h, cnt.l1f, cnt.l0f);
+                             sc.max_shadow, h, cnt.l1f, cnt.l0f)) {
+      if constexpr (kDefer)
+        deferred = !dense_run_prefetch(r, lk, w, h, kShadowPauseSteps);
+      else
+        dense_run<kStats>(r, lk, w, h, cnt.l1f, cnt.l0f);
+    }
     shit = h.hit;
     if (shit) {
       shx = h.hx;
@@ -374,6 +471,26 @@ __device__ __forceinline__ void trace_pixel_dense(const DenseWorld& w, const Vie
       shz = h.hz;
     }
     ssteps = h.steps;
+  }
+  if constexpr (kDefer) {
+    ShadowJob job;
+    job.pix = py * v.W + px;
+    job.wx = r.wx;
+    job.wy = r.wy;
+    job.wz = r.wz;
+    job.c = r.c;
+    job.steps = r.steps;
+    job.data = p.data;
+    job.info = hit_info(p.face, p.sub, p.material, false);
+    job.psteps = (uint32_t)p.steps;
+    job.sox = sox;
+    job.soy = soy;
+    job.soz = soz;
+    enqueue_shadow_job(*q, deferred, job);
+    if (deferred) {
+      write_pixel_deferred(v, sc, out, (size_t)py * v.W + px, p);
+      return;
+    }
   }
   write_pixel<kShade>(v, sc, out, px, py, p, shit, shx, shy, shz, sox, soy, soz, ssteps, cnt, &w,
                      gacc);

@@ -314,14 +314,11 @@ struct DenseRay {
   int steps, max_steps;
 };
 
-// Set-up (map.glsl:85-104) plus iteration 0 (stepSize 0).  Returns true when
-// the ray already terminated (hit or miss) in iteration 0; otherwise the ray
-// is at the block level, ready for the DDA step of iteration 0.
-template <bool kStats, class L>
-__device__ __forceinline__ bool dense_begin(DenseRay& r, L& lk, const DenseWorld& w, float ox,
-                                            float oy, float oz, float dx, float dy, float dz,
-                                            int max_steps, DenseHit& h, uint32_t& l1f,
-                                            uint32_t& l0f) {
+// The per-ray constants of traceMap's set-up (map.glsl:85-97): the 0.001
+// substitution of zero direction components, positivity, 1/d, and the
+// block-level crossing and re-anchoring constants.
+__device__ __forceinline__ void dense_ray_consts(DenseRay& r, const DenseWorld& w, float dx,
+                                                 float dy, float dz) {
   if (dx == 0.0f) dx = 0.001f;
   if (dy == 0.0f) dy = 0.001f;
   if (dz == 0.0f) dz = 0.001f;
@@ -344,6 +341,17 @@ __device__ __forceinline__ bool dense_begin(DenseRay& r, L& lk, const DenseWorld
   r.Ay = (float)((1 - r.py) << 3) * 0.999f;
   r.Az = (float)((1 - r.pz) << 3) * 0.999f;
   r.flip = (r.px ? 0u : w.mx) | (r.py ? 0u : w.my) | (r.pz ? 0u : w.mz);
+}
+
+// Set-up (map.glsl:85-104) plus iteration 0 (stepSize 0).  Returns true when
+// the ray already terminated (hit or miss) in iteration 0; otherwise the ray
+// is at the block level, ready for the DDA step of iteration 0.
+template <bool kStats, class L>
+__device__ __forceinline__ bool dense_begin(DenseRay& r, L& lk, const DenseWorld& w, float ox,
+                                            float oy, float oz, float dx, float dy, float dz,
+                                            int max_steps, DenseHit& h, uint32_t& l1f,
+                                            uint32_t& l0f) {
+  dense_ray_consts(r, w, dx, dy, dz);
   r.steps = 0;
   r.max_steps = max_steps;
   lk.reset();
@@ -546,18 +554,22 @@ __device__ __forceinline__ void dense_loop_prefetch(
 // steps, so the result is unchanged.  What this buys: one divergent exit per
 // K steps instead of one per step, and with it none of the per-step copies of
 // the exit state that a divergent exit costs.  An iteration only starts when
-// steps + K < max_steps, so the step cap falls inside the replay.
+// steps + K < limit (limit <= max_steps), so the step cap falls inside the
+// replay.  Returns true when it stopped at an iteration that met a nonzero
+// byte or left the grid, false when it stopped at the limit; either way `s`
+// is a state the exact loop (or another call) continues from.
 template <int K>
-__device__ __forceinline__ void dense_skip_empty(float Px, float Py, float Pz, float Ax, float Ay,
+__device__ __forceinline__ bool dense_skip_empty(float Px, float Py, float Pz, float Ax, float Ay,
                                                  float Az, float dx, float dy, float dz,
                                                  float ix, float iy, float iz, uint32_t mx,
                                                  uint32_t my, uint32_t mz, uint32_t flip,
-                                                 DenseCell& s, int& steps, int max_steps,
+                                                 DenseCell& s, int& steps, int limit,
                                                  __amdgpu_buffer_rsrc_t bytes) {
   float bx = s.wx, by = s.wy, bz = s.wz;
   uint32_t bc = s.c;
   int sb = steps;
-  while (sb + K < max_steps) {
+  bool blocked = false;
+  while (sb + K < limit) {
     DenseCell n;
     n.wx = bx;
     n.wy = by;
@@ -571,7 +583,10 @@ __device__ __forceinline__ void dense_skip_empty(float Px, float Py, float Pz, f
       any |= (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(bytes, (int)n.a, 0, 0);
       wrapped |= n.wrapped;
     }
-    if (wrapped | (any != 0u)) break;
+    if (wrapped | (any != 0u)) {
+      blocked = true;
+      break;
+    }
     bx = n.wx;
     by = n.wy;
     bz = n.wz;
@@ -583,11 +598,18 @@ __device__ __forceinline__ void dense_skip_empty(float Px, float Py, float Pz, f
   s.wz = bz;
   s.c = bc;
   steps = sb;
+  return blocked;
 }
 
+// The flat lookup's traversal from r's block-level state: the empty-space
+// phase, then the exact loop.  With pause_at < r.max_steps the empty-space
+// phase stops at that step count instead; if it got there without meeting a
+// nonzero byte or leaving the grid, the ray is left unfinished: r holds the
+// state it continues from (within, code, steps) and the call returns false.
+// Otherwise the ray is finished into h and the call returns true.
 template <class L>
-__device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const DenseWorld& w,
-                                                   DenseHit& h) {
+__device__ __forceinline__ bool dense_run_prefetch(DenseRay& r, L& lk, const DenseWorld& w,
+                                                   DenseHit& h, int pause_at) {
   const uint32_t mx = in_vgpr(w.mx), my = in_vgpr(w.my), mz = in_vgpr(w.mz), flip = r.flip;
   const int max_steps = r.max_steps;
   DenseCell s0;
@@ -598,8 +620,18 @@ __device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const Den
   const __amdgpu_buffer_rsrc_t vox =
       __builtin_amdgcn_make_buffer_rsrc((void*)w.vox, (short)0, (int)w.nvox, 0x00020000);
   int steps0 = r.steps;
-  dense_skip_empty<VT_SKIP_STEPS>(r.Px, r.Py, r.Pz, r.Ax, r.Ay, r.Az, r.dx, r.dy, r.dz, r.ix, r.iy,
-                                  r.iz, mx, my, mz, flip, s0, steps0, max_steps, vox);
+  const int limit = pause_at < max_steps ? pause_at : max_steps;
+  const bool blocked =
+      dense_skip_empty<VT_SKIP_STEPS>(r.Px, r.Py, r.Pz, r.Ax, r.Ay, r.Az, r.dx, r.dy, r.dz, r.ix,
+                                      r.iy, r.iz, mx, my, mz, flip, s0, steps0, limit, vox);
+  if (!blocked && limit < max_steps) {
+    r.wx = s0.wx;
+    r.wy = s0.wy;
+    r.wz = s0.wz;
+    r.c = s0.c;
+    r.steps = steps0;
+    return false;
+  }
   float fwx, fwy, fwz;
   uint32_t fc, fM, fa, fv;
   int sF;
@@ -611,7 +643,7 @@ __device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const Den
   if (sF >= max_steps || wrappedF) {  // the loop cap, then the bounds test
     h.hit = false;
     h.steps = sF;
-    return;
+    return true;
   }
   const uint32_t g = fc ^ flip;
   // faceId = 2 minIdx + 2 - rayPositivity[minIdx] (map.glsl:119-125)
@@ -630,13 +662,14 @@ __device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const Den
   h.hy = (float)(compact1by2(g >> 1) << 3) + fwy;
   h.hz = (float)(compact1by2(g >> 2) << 3) + fwz;
   h.steps = sF + 1;
+  return true;
 }
 
 template <bool kStats, class L>
 __device__ __forceinline__ void dense_run(DenseRay& r, L& lk, const DenseWorld& w, DenseHit& h,
                                           uint32_t& l1f, uint32_t& l0f) {
   if constexpr (L::kPrefetch && !kStats) {
-    dense_run_prefetch(r, lk, w, h);
+    dense_run_prefetch(r, lk, w, h, r.max_steps);
     return;
   }
   const float Px = r.Px, Py = r.Py, Pz = r.Pz, Ax = r.Ax, Ay = r.Ay, Az = r.Az;
