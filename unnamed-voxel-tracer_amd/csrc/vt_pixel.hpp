@@ -6,11 +6,21 @@
 
 namespace vt {
 
+// Block id -> image block.  Blocks are dealt round-robin over the 8 XCDs
+// (hardware id b runs on XCD b % 8); each XCD should see runs of neighbouring
+// image blocks (their rays share cells in that XCD's L2) without one XCD
+// getting a much costlier part of the image than another (a band of sky
+// costs a few steps per ray, a band of near terrain hundreds).  So the image
+// blocks, in row-major order, are cut into strips of kXcdStrip blocks dealt
+// to the XCDs in turn: XCD x traces strips x, x + 8, x + 16, ...  Ids past the
+// last full round of 8 strips map to themselves.
+constexpr uint32_t kXcdStrip = 16u;
+
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t n) {
-  const uint32_t X = 8u;
-  uint32_t q = n / X, r = n % X;
-  uint32_t x = bid % X, j = bid / X;
-  return x < r ? x * (q + 1u) + j : r * (q + 1u) + (x - r) * q + j;
+  const uint32_t X = 8u, round = X * kXcdStrip;
+  if (bid >= (n / round) * round) return bid;
+  const uint32_t x = bid % X, j = bid / X;
+  return ((j / kXcdStrip) * X + x) * kXcdStrip + j % kXcdStrip;
 }
 
 struct Primary {

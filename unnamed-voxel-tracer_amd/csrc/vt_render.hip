@@ -9,9 +9,10 @@
 //
 // Launch shape: 256-thread blocks = 4 wave64s; each wave owns one tile of 64
 // pixels (8x8, 16x4 or 32x2) for ray coherence, a block 2x2 tiles.  Block ids
-// are remapped so that each XCD (blocks are dealt round-robin over the 8 XCDs)
-// receives a contiguous band of the image, keeping each XCD's L2 working set
-// local.  Each lane traces its primary ray, then (on a hit) its shadow ray.
+// are remapped (xcd_remap) so that each XCD receives strips of neighbouring
+// blocks spread over the whole image: its L2 working set stays local and
+// no XCD is left with the costliest rows.  Each lane traces its primary ray,
+// then (on a hit) its shadow ray.
 //
 // Shadow rays come in two kinds: most of those not lit are occluded after a
 // step or two, most lit ones run to the edge of the grid (hundreds of steps),
