@@ -35,12 +35,12 @@ def _frame_dict(f):
 
 
 def gpu_render(ctx, world, cam, W, H, max_p, max_s, flags=0, traversal=0, ambient=0.0,
-               albedo=None):
+               albedo=None, sun=S.SUN):
     """Renders twice: with the statistics counters (the kernel variant that
     counts every lookup) and without (the variant the benchmark times, with
     its own traversal loops); compare() checks both against the oracle."""
     import torch
-    prm = vt.params(W, H, max_p, max_s, S.SUN, flags, traversal, ambient, albedo)
+    prm = vt.params(W, H, max_p, max_s, sun, flags, traversal, ambient, albedo)
     f = vt.Frame.alloc(W, H, gbuffer=True, steps=True, stats=True)
     vt.render(ctx, world, cam, f, prm)
     ff = vt.Frame.alloc(W, H, gbuffer=True, steps=True)
@@ -90,6 +90,22 @@ def test_c0_full_frame(ctx, c0, traversal, flags):
     g = gpu_render(ctx, w, cam, W, H, 512, 512, flags=flags, traversal=traversal)
     o = O.render(ow, cam, W, H, 512, 512, S.SUN, flags=flags, traversal=traversal)
     compare(g, o, label=f"c0 traversal={traversal} flags={flags}")
+    w.close()
+
+
+@pytest.mark.parametrize("sun", [(-0.5, 0.7, -0.2), (0.0, 1.0, 0.0), (0.6, 0.5, -0.6),
+                                 (-0.3, -0.9, 0.1)])
+def test_c0_sun_directions(ctx, c0, sun):
+    """Shadow rays along other suns: components of either sign and zero (the
+    0.001 substitution); the shadow pass has a separate loop for suns with
+    no negative component."""
+    grid, pal = c0
+    w = vt.World.dense(ctx, grid, pal)
+    ow = O.DenseWorld(grid, pal)
+    cam, W, H = S.views("c0")[0]
+    g = gpu_render(ctx, w, cam, W, H, 512, 512, sun=sun)
+    o = O.render(ow, cam, W, H, 512, 512, sun)
+    compare(g, o, label=f"c0 sun={sun}")
     w.close()
 
 

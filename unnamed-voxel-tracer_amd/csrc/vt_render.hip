@@ -58,6 +58,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
 // the x-th eighth of the list (neighbouring jobs are neighbouring pixels,
 // whose rays share cells in that XCD's L2), its blocks striding through it.
 // The grid is a multiple of 8 blocks.
+// kPositive: every component of the sun direction is >= 0 (after the zero
+// substitution all are positive), which selects dense_skip_positive.
+template <bool kPositive>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_shadow_finish(
     DenseWorld w, ViewConsts v, ShadeConsts sc, FrameOut out, ShadowQueue q) {
   const uint32_t n = *q.count;
@@ -77,7 +80,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     r.max_steps = sc.max_shadow;
     FlatLookup lk;
     DenseHit h;
-    dense_run_prefetch(r, lk, w, h, sc.max_shadow);
+    dense_run_prefetch<FlatLookup, kPositive>(r, lk, w, h, sc.max_shadow);
     write_pixel_shadow(v, sc, out, j, h);
   }
 }
@@ -196,8 +199,12 @@ hipError_t launch_render_dense(const DenseWorld& w, uint32_t traversal, const Vi
                        stream, w, v, sc, out, tile.log2_w, blocks_x, *queue);
     const uint32_t npix = v.W * v.H;
     const uint32_t fblocks = std::min<uint32_t>(((npix + 255u) / 256u + 7u) & ~7u, 2048u);
-    hipLaunchKernelGGL(k_shadow_finish, dim3(fblocks), dim3(256), 0, stream, w, v, sc, out,
-                       *queue);
+    if (sc.sun[0] >= 0.0f && sc.sun[1] >= 0.0f && sc.sun[2] >= 0.0f)
+      hipLaunchKernelGGL(k_shadow_finish<true>, dim3(fblocks), dim3(256), 0, stream, w, v, sc,
+                         out, *queue);
+    else
+      hipLaunchKernelGGL(k_shadow_finish<false>, dim3(fblocks), dim3(256), 0, stream, w, v, sc,
+                         out, *queue);
     return hipGetLastError();
   }
   if (traversal == VT_TRAVERSAL_BITMASK) {

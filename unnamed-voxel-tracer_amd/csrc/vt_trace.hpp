@@ -601,13 +601,102 @@ __device__ __forceinline__ bool dense_skip_empty(float Px, float Py, float Pz, f
   return blocked;
 }
 
+// The empty-space phase for rays whose direction is positive on every axis
+// (after the zero substitution) -- the shadow rays of a sun with no negative
+// component.  Then rayPositivity is 1 everywhere: the crossing position is
+// 8, the re-anchored within is 0 and the Morton code is the address (no
+// complemented axes).  Same contract as dense_skip_empty; what differs is how
+// the lookup of a state S_i is made exact without a branch:
+//   pos >> 3 = G + (int(within) >> 3) moves the probed block one step along
+//   each axis k with within_k >= 8, and with positivity 1 that is exactly
+//   t_k = (8 - within_k) / d_k <= 0 for the next step.  When one axis has
+//   t <= 0 it is the smallest crossing time, so the next step crosses that
+//   axis and lands in the very block the corrected lookup probes: the lookup
+//   of S_i is the block of S_{i+1}.  Two or more axes at t <= 0 (the median
+//   crossing time <= 0) stop the iteration, and the exact loop replays it.
+// The lookup of an iteration's last state has no next step inside the
+// iteration and takes the general branch (dense_step's).  Shadow rays start
+// on lattice points, so this correction is frequent for them: as a branch it
+// ran in about half of the wave-steps.
+template <int K>
+__device__ __forceinline__ bool dense_skip_positive(float dx, float dy, float dz, float ix,
+                                                    float iy, float iz, uint32_t mx, uint32_t my,
+                                                    uint32_t mz, DenseCell& s, int& steps,
+                                                    int limit, __amdgpu_buffer_rsrc_t bytes) {
+  float bx = s.wx, by = s.wy, bz = s.wz;
+  uint32_t bc = s.c;
+  int sb = steps;
+  bool blocked = false;
+  while (sb + K < limit) {
+    float wx = bx, wy = by, wz = bz;
+    uint32_t c = bc;
+    uint32_t any = 0u;
+    bool stop = false;
+    float med = 1.0f;  // smallest median crossing time of the iteration
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      // the step from S_i (S_0 is the iteration's start, already probed)
+      const float tx = (8.0f - wx) * ix;
+      const float ty = (8.0f - wy) * iy;
+      const float tz = (8.0f - wz) * iz;
+      const bool xy = tx < ty, xz = tx < tz, yz = ty < tz;
+      const bool m0 = xy & xz;
+      const bool m1 = yz & !m0;
+      const float tm = fminf(fminf(tx, ty), tz);
+      const float nwx = wx + dx * tm;
+      const float nwy = wy + dy * tm;
+      const float nwz = wz + dz * tm;
+      wx = m0 ? 0.0f : nwx;
+      wy = m1 ? 0.0f : nwy;
+      wz = (m0 | m1) ? nwz : 0.0f;
+      const uint32_t M = m0 ? mx : (m1 ? my : mz);
+      const uint32_t cm = c & M;
+      stop |= cm == M;
+      const uint32_t cn = ((cm - M) & M) | (c & ~M);
+      if (i > 0) {  // the lookup of S_i
+        any |= (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(bytes, (int)(tm <= 0.0f ? cn : c), 0,
+                                                             0);
+        med = fminf(med, __builtin_amdgcn_fmed3f(tx, ty, tz));
+      }
+      c = cn;
+    }
+    stop |= med <= 0.0f;
+    {  // the lookup of S_K
+      uint32_t a = c;
+      if (fmaxf(fmaxf(wx, wy), wz) >= 8.0f) {
+        if (wx >= 8.0f) a = morton_inc_or_out(a, mx);
+        if (wy >= 8.0f) a = morton_inc_or_out(a, my);
+        if (wz >= 8.0f) a = morton_inc_or_out(a, mz);
+      }
+      any |= (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(bytes, (int)a, 0, 0);
+    }
+    if (stop | (any != 0u)) {
+      blocked = true;
+      break;
+    }
+    bx = wx;
+    by = wy;
+    bz = wz;
+    bc = c;
+    sb += K;
+  }
+  s.wx = bx;
+  s.wy = by;
+  s.wz = bz;
+  s.c = bc;
+  steps = sb;
+  return blocked;
+}
+
 // The flat lookup's traversal from r's block-level state: the empty-space
 // phase, then the exact loop.  With pause_at < r.max_steps the empty-space
 // phase stops at that step count instead; if it got there without meeting a
 // nonzero byte or leaving the grid, the ray is left unfinished: r holds the
 // state it continues from (within, code, steps) and the call returns false.
 // Otherwise the ray is finished into h and the call returns true.
-template <class L>
+// kPositive: the caller guarantees a direction positive on every axis (after
+// the zero substitution), and the empty-space phase is dense_skip_positive.
+template <class L, bool kPositive = false>
 __device__ __forceinline__ bool dense_run_prefetch(DenseRay& r, L& lk, const DenseWorld& w,
                                                    DenseHit& h, int pause_at) {
   const uint32_t mx = in_vgpr(w.mx), my = in_vgpr(w.my), mz = in_vgpr(w.mz), flip = r.flip;
@@ -621,9 +710,14 @@ __device__ __forceinline__ bool dense_run_prefetch(DenseRay& r, L& lk, const Den
       __builtin_amdgcn_make_buffer_rsrc((void*)w.vox, (short)0, (int)w.nvox, 0x00020000);
   int steps0 = r.steps;
   const int limit = pause_at < max_steps ? pause_at : max_steps;
-  const bool blocked =
-      dense_skip_empty<VT_SKIP_STEPS>(r.Px, r.Py, r.Pz, r.Ax, r.Ay, r.Az, r.dx, r.dy, r.dz, r.ix,
-                                      r.iy, r.iz, mx, my, mz, flip, s0, steps0, limit, vox);
+  bool blocked;
+  if constexpr (kPositive)
+    blocked = dense_skip_positive<VT_SKIP_STEPS>(r.dx, r.dy, r.dz, r.ix, r.iy, r.iz, mx, my, mz,
+                                                 s0, steps0, limit, vox);
+  else
+    blocked = dense_skip_empty<VT_SKIP_STEPS>(r.Px, r.Py, r.Pz, r.Ax, r.Ay, r.Az, r.dx, r.dy,
+                                              r.dz, r.ix, r.iy, r.iz, mx, my, mz, flip, s0, steps0,
+                                              limit, vox);
   if (!blocked && limit < max_steps) {
     r.wx = s0.wx;
     r.wy = s0.wy;
