@@ -248,13 +248,8 @@ int vt_world_info(const vt_world* world, uint32_t* dim, size_t* device_bytes);
 /* ---- render ------------------------------------------------------------ */
 /* Primary pass (primary.comp.glsl:23-69) + shadow pass
  * (secondary.comp.glsl:18-50) + blit composite formula
- * (blit.fragment.glsl:30-31), replacing the dispatches of Game.render
- * (game.zig:241-248).  Dense worlds with the flat traversal, the reference
- * composite and no statistics run a frame kernel and then a shadow pass over
- * the shadow rays still running after a few steps; the context holds their
- * job list (48 B per pixel of the largest frame, allocated on first use), so
- * calls on one context must not overlap on different streams.  Everything
- * else is one launch.                                                      */
+ * (blit.fragment.glsl:30-31), fused into one launch; replaces the
+ * dispatches of Game.render (game.zig:241-248).                           */
 int vt_render(vt_ctx* ctx, const vt_world* world, const vt_camera* camera,
               const vt_render_params* params, const vt_frame* frame, void* stream);
 

@@ -258,9 +258,6 @@ int vt_create(vt_ctx** out, int device) {
   if (!c) return fail(VT_ERR_OUT_OF_MEMORY, "vt_create: host allocation");
   c->device = device;
   c->compute_units = prop.multiProcessorCount;
-  c->shadow_jobs = nullptr;
-  c->shadow_jobs_cap = 0;
-  c->shadow_count = nullptr;
   std::snprintf(c->name, sizeof c->name, "%s", prop.gcnArchName);
   *out = c;
   return VT_OK;
@@ -268,12 +265,6 @@ int vt_create(vt_ctx** out, int device) {
 
 int vt_destroy(vt_ctx* ctx) {
   if (!ctx) return fail(VT_ERR_NULL_POINTER, "vt_destroy: ctx is NULL");
-  if (ctx->shadow_jobs || ctx->shadow_count) {
-    (void)hipSetDevice(ctx->device);
-    (void)hipDeviceSynchronize();
-    vt::dev_free(ctx->shadow_jobs);
-    vt::dev_free(ctx->shadow_count);
-  }
   delete ctx;
   return VT_OK;
 }
@@ -420,31 +411,6 @@ int vt_world_info(const vt_world* world, uint32_t* dim, size_t* device_bytes) {
   return VT_OK;
 }
 
-// The context's job list for deferred shadow rays, grown to n entries.  A
-// larger frame than any before waits for the device before the old list is
-// freed (it may still be in use by an earlier call).
-static int ensure_shadow_jobs(vt_ctx* ctx, size_t n) {
-  if (ctx->shadow_jobs_cap >= n) return VT_OK;
-  VT_HIP(hipDeviceSynchronize());
-  vt::dev_free(ctx->shadow_jobs);
-  ctx->shadow_jobs = nullptr;
-  ctx->shadow_jobs_cap = 0;
-  if (!ctx->shadow_count) {
-    hipError_t e = hipMalloc((void**)&ctx->shadow_count, sizeof(uint32_t));
-    if (e != hipSuccess) {
-      ctx->shadow_count = nullptr;
-      return hip_fail(e, "vt_render: shadow job counter");
-    }
-  }
-  hipError_t e = hipMalloc(&ctx->shadow_jobs, n * sizeof(vt::ShadowJob));
-  if (e != hipSuccess) {
-    ctx->shadow_jobs = nullptr;
-    return hip_fail(e, "vt_render: shadow job list");
-  }
-  ctx->shadow_jobs_cap = n;
-  return VT_OK;
-}
-
 int vt_render(vt_ctx* ctx, const vt_world* world, const vt_camera* camera,
               const vt_render_params* params, const vt_frame* frame, void* stream) {
   if (!ctx || !world || !camera || !params || !frame)
@@ -461,16 +427,9 @@ int vt_render(vt_ctx* ctx, const vt_world* world, const vt_camera* camera,
   const vt::FrameOut o = vt::frame_out(frame);
   vt::TileShape tile{3};
   hipError_t e;
-  if (world->kind == WorldKind::Dense) {
-    vt::ShadowQueue q{nullptr, nullptr};
-    if (vt::render_defers_shadows(p.traversal, sc, o)) {
-      if ((st = ensure_shadow_jobs(ctx, (size_t)p.width * p.height)) != VT_OK) return st;
-      q.jobs = (vt::ShadowJob*)ctx->shadow_jobs;
-      q.count = ctx->shadow_count;
-    }
+  if (world->kind == WorldKind::Dense)
     e = vt::launch_render_dense(vt::dense_view(world), p.traversal, v, sc, o, tile,
-                                q.jobs ? &q : nullptr, (hipStream_t)stream);
-  }
+                                (hipStream_t)stream);
   else
     e = vt::launch_render_brickmap(vt::brickmap_view(world), v, sc, o, tile, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "vt_render: launch");

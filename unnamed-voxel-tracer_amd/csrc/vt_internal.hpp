@@ -13,10 +13,6 @@ struct vt_ctx {
   int device;
   int compute_units;
   char name[256];
-  // deferred shadow rays of vt_render (grown to the largest frame rendered)
-  void* shadow_jobs;
-  size_t shadow_jobs_cap;  // entries
-  uint32_t* shadow_count;
 };
 
 enum class WorldKind { Dense, Brickmap };
@@ -85,32 +81,6 @@ struct alignas(8) FrameOut {
   uint32_t* display;     // VT_FLAG_PRESENT RGBA8 output
 };
 
-// A shadow ray the frame kernel left unfinished after its first steps (the
-// rays that are not occluded within a few steps run for hundreds), finished
-// by the shadow pass together with the other long ones.  Holds the ray's
-// block-level state and what the shadow-dependent stores of its pixel need.
-struct alignas(16) ShadowJob {
-  uint32_t pix;        // y * W + x
-  float wx, wy, wz;    // withinGridCoords at the block level
-  uint32_t c;          // flipped Morton code of the block
-  int steps;           // steps done
-  uint32_t data;       // primary hit colour (RGBA8)
-  uint32_t info;       // hit-record info word without the shadow bit
-  uint32_t psteps;     // primary steps
-  float sox, soy, soz; // shadow ray origin
-};
-static_assert(sizeof(ShadowJob) == 48, "ShadowJob layout");
-
-// The job list of one vt_render call: capacity >= W * H, *count zeroed
-// before the frame kernel.
-struct ShadowQueue {
-  ShadowJob* jobs;
-  uint32_t* count;
-};
-
-// Steps a shadow ray takes in the frame kernel before it is deferred.
-constexpr int kShadowPauseSteps = 10;
-
 // Wave tile: 64 lanes cover tile_w x (64 / tile_w) pixels; a 256-thread
 // block covers 2 x 2 wave tiles.
 struct TileShape {
@@ -128,14 +98,9 @@ ViewConsts view_consts(const vt_camera* camera, uint32_t W, uint32_t H);
 ShadeConsts shade_consts(const vt_render_params& p);
 FrameOut frame_out(const vt_frame* frame);
 
-// `queue` (may be null): job list for the deferred shadow pass, used when
-// the launch is the flat traversal with the reference composite and no
-// statistics.
 hipError_t launch_render_dense(const DenseWorld& w, uint32_t traversal, const ViewConsts& v,
                                const ShadeConsts& sc, const FrameOut& out, TileShape tile,
-                               const ShadowQueue* queue, hipStream_t stream);
-// Whether launch_render_dense uses the job list for these parameters.
-bool render_defers_shadows(uint32_t traversal, const ShadeConsts& sc, const FrameOut& out);
+                               hipStream_t stream);
 hipError_t launch_render_brickmap(const BrickmapWorld& w, const ViewConsts& v,
                                   const ShadeConsts& sc, const FrameOut& out, TileShape tile,
                                   hipStream_t stream);

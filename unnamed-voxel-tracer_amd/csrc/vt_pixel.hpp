@@ -351,43 +351,6 @@ __device__ __forceinline__ void write_pixel(const ViewConsts& v, const ShadeCons
   }
 }
 
-// Stores of a hit pixel whose shadow ray was deferred: the ones that do not
-// depend on it (colour, normal, position, hit-record voxel and distance).
-__device__ __forceinline__ void write_pixel_deferred(const ViewConsts& v, const ShadeConsts& sc,
-                                                     const FrameOut& out, size_t pix,
-                                                     const Primary& p) {
-  const bool unorm = (sc.flags & VT_FLAG_GBUF_UNORM8) != 0u;
-  float4 nrm, pos;
-  float t;
-  hit_normal_position(v, p, unorm, nrm, pos, t);
-  if (out.color) out.color[pix] = hit_color(p.data, unorm);
-  if (out.normal) out.normal[pix] = nrm;
-  if (out.position) out.position[pix] = pos;
-  if (out.hits) {
-    out.hits[3 * pix + 0] = p.voxel;
-    out.hits[3 * pix + 1] = __float_as_uint(t);
-  }
-}
-
-// The remaining stores of a deferred pixel once its shadow ray is finished
-// (h): illumination, step counts, hit-record info word, composite.
-__device__ __forceinline__ void write_pixel_shadow(const ViewConsts& v, const ShadeConsts& sc,
-                                                   const FrameOut& out, const ShadowJob& j,
-                                                   const DenseHit& h) {
-  const size_t pix = j.pix;
-  const bool unorm = (sc.flags & VT_FLAG_GBUF_UNORM8) != 0u;
-  const float shx = h.hit ? h.hx : -1.0f, shy = h.hit ? h.hy : -1.0f,
-              shz = h.hit ? h.hz : -1.0f;
-  bool ent, occl;
-  const float4 ill = shadow_illum(sc, unorm, h.hit, shx, shy, shz, j.sox, j.soy, j.soz, ent, occl);
-  if (out.illum) out.illum[pix] = ill;
-  if (out.steps) out.steps[pix] = j.psteps | ((uint32_t)h.steps << 16);
-  if (out.hits) out.hits[3 * pix + 2] = j.info | ((occl ? 1u : 0u) << 3);
-  if (out.shaded)
-    store_ref_composite(v, sc, out, pix, j.pix % v.W, j.pix / v.W, hit_color(j.data, unorm), ill,
-                        true);
-}
-
 __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
@@ -403,34 +366,13 @@ __device__ __forceinline__ void flush_counters(const FrameOut& out, const Counte
   }
 }
 
-// Appends the deferred shadow rays of a wave to the job list: one atomic per
-// wave, slots in lane order.  Every lane of the wave that traced a pixel must
-// call this.
-__device__ __forceinline__ void enqueue_shadow_job(const ShadowQueue& q, bool defer,
-                                                   const ShadowJob& job) {
-  const unsigned long long m = __ballot(defer);
-  if (m == 0ull) return;
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t leader = (uint32_t)__ffsll(m) - 1u;
-  uint32_t base = 0u;
-  if (lane == leader) base = atomicAdd(q.count, (uint32_t)__popcll(m));
-  base = __shfl(base, (int)leader, 64);
-  if (defer) q.jobs[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = job;
-}
-
 // One pixel of a dense world: primary ray (primary.comp.glsl:31-68), then the
-// shadow ray of a hit (secondary.comp.glsl:25-50), then every store.  With
-// kDefer (flat lookup, reference composite, no statistics) a shadow ray still
-// running after kShadowPauseSteps steps goes to the job list `q` instead: the
-// pixel's shadow-independent stores are made here, the rest by the shadow
-// pass (k_shadow_finish) once the ray is finished.
-template <class L, bool kStats, int kShade, bool kDefer = false>
+// shadow ray of a hit (secondary.comp.glsl:25-50), then every store.
+template <class L, bool kStats, int kShade>
 __device__ __forceinline__ void trace_pixel_dense(const DenseWorld& w, const ViewConsts& v,
                                                   const ShadeConsts& sc, const FrameOut& out,
                                                   uint32_t px, uint32_t py, Counters& cnt,
-                                                  GradAcc& gacc, const ShadowQueue* q = nullptr) {
-  static_assert(!kDefer || (L::kPrefetch && !kStats && kShade == kShadeRef),
-                "shadow deferral: flat lookup, reference composite, no statistics");
+                                                  GradAcc& gacc) {
   DenseRay r;
   L lk;
   DenseHit h;
@@ -454,7 +396,6 @@ __device__ __forceinline__ void trace_pixel_dense(const DenseWorld& w, const Vie
   bool shit = false;
   float shx = -1.0f, shy = -1.0f, shz = -1.0f;
   int ssteps = 0;
-  bool deferred = false;
   if (p.hit) {
     p.face = h.face;
     p.voxel = (uint32_t)h.bx + w.dim * ((uint32_t)h.by + w.dim * (uint32_t)h.bz);
@@ -469,10 +410,15 @@ __device__ __forceinline__ void trace_pixel_dense(const DenseWorld& w, const Vie
     shadow_origin(p, sc.flags, sox, soy, soz);
     if (!dense_begin<kStats>(r, lk, w, sox, soy, soz, sc.sun[0], sc.sun[1], sc.sun[2],
                              sc.max_shadow, h, cnt.l1f, cnt.l0f)) {
-      if constexpr (kDefer)
-        deferred = !dense_run_prefetch(r, lk, w, h, kShadowPauseSteps);
-      else
+      if constexpr (L::kPrefetch && !kStats) {
+        // the sun is the same for every lane: a uniform branch
+        if (sc.sun[0] >= 0.0f && sc.sun[1] >= 0.0f && sc.sun[2] >= 0.0f)
+          dense_run_prefetch<L, true>(r, lk, w, h);
+        else
+          dense_run_prefetch<L, false>(r, lk, w, h);
+      } else {
         dense_run<kStats>(r, lk, w, h, cnt.l1f, cnt.l0f);
+      }
     }
     shit = h.hit;
     if (shit) {
@@ -481,26 +427,6 @@ __device__ __forceinline__ void trace_pixel_dense(const DenseWorld& w, const Vie
       shz = h.hz;
     }
     ssteps = h.steps;
-  }
-  if constexpr (kDefer) {
-    ShadowJob job;
-    job.pix = py * v.W + px;
-    job.wx = r.wx;
-    job.wy = r.wy;
-    job.wz = r.wz;
-    job.c = r.c;
-    job.steps = r.steps;
-    job.data = p.data;
-    job.info = hit_info(p.face, p.sub, p.material, false);
-    job.psteps = (uint32_t)p.steps;
-    job.sox = sox;
-    job.soy = soy;
-    job.soz = soz;
-    enqueue_shadow_job(*q, deferred, job);
-    if (deferred) {
-      write_pixel_deferred(v, sc, out, (size_t)py * v.W + px, p);
-      return;
-    }
   }
   write_pixel<kShade>(v, sc, out, px, py, p, shit, shx, shy, shz, sox, soy, soz, ssteps, cnt, &w,
                      gacc);

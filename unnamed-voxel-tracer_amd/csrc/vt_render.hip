@@ -15,24 +15,21 @@
 // then (on a hit) its shadow ray.
 //
 // Shadow rays come in two kinds: most of those not lit are occluded after a
-// step or two, most lit ones run to the edge of the grid (hundreds of steps),
-// so a wave whose pixels mix both waits on its longest lane for most of the
-// shadow phase.  With the reference composite (the timed path) a shadow ray
-// still running after kShadowPauseSteps steps is therefore deferred: its
-// state goes to a job list, and a second kernel (k_shadow_finish) runs the
-// long rays packed 64 to a wave and makes their pixels' remaining stores.
-#include <algorithm>
-
+// step or two, most lit ones run to the edge of the grid (hundreds of steps).
+// A second kernel over the long ones alone (a job list, 64 long rays to a
+// wave) was built and measured: with the XCD strips it was slower than
+// this single kernel at every frame size from 256^2 to 2560x1440, so the
+// shadow ray stays in the pixel's lane.
 #include "vt_pixel.hpp"
 
 namespace vt {
 
 // amdgpu_waves_per_eu(8): at most 64 VGPRs, so eight waves per SIMD hide the
 // gathers of the traversal loop (the few spills it costs lie outside it).
-template <class L, bool kStats, int kShade, bool kDefer = false>
+template <class L, bool kStats, int kShade>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_render_dense(
     DenseWorld w, ViewConsts v, ShadeConsts sc, FrameOut out, uint32_t log2_tw,
-    uint32_t blocks_x, ShadowQueue q) {
+    uint32_t blocks_x) {
   const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
   const uint32_t tw = 1u << log2_tw, th = 64u >> log2_tw;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -42,51 +39,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   Counters cnt = {0, 0, 0, 0, 0, 0, 0, 0};
   GradAcc gacc = grad_acc();
   if (px < v.W && py < v.H)
-    trace_pixel_dense<L, kStats, kShade, kDefer>(w, v, sc, out, px, py, cnt, gacc, &q);
+    trace_pixel_dense<L, kStats, kShade>(w, v, sc, out, px, py, cnt, gacc);
   if (kStats) flush_counters(out, cnt);
   if constexpr (kShade == kShadeLambertGrad) {
     flush_albedo(sc, gacc);
     flush_grad(sc, gacc);
   }
-}
-
-// The shadow pass: each deferred shadow ray continued from its saved state to
-// the end (same steps and stopping rule as an undeferred one), then the
-// stores of its pixel that depend on it.  The job count is only known on the
-// device (*q.count), so the grid is sized for the whole image and the jobs are
-// split here: blocks are dealt round-robin over the 8 XCDs, and XCD x takes
-// the x-th eighth of the list (neighbouring jobs are neighbouring pixels,
-// whose rays share cells in that XCD's L2), its blocks striding through it.
-// The grid is a multiple of 8 blocks.
-// kPositive: every component of the sun direction is >= 0 (after the zero
-// substitution all are positive), which selects dense_skip_positive.
-template <bool kPositive>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_shadow_finish(
-    DenseWorld w, ViewConsts v, ShadeConsts sc, FrameOut out, ShadowQueue q) {
-  const uint32_t n = *q.count;
-  const uint32_t X = 8u;
-  const uint32_t x = blockIdx.x % X, jb = blockIdx.x / X, per_x = gridDim.x / X;
-  const uint32_t share = (n + X - 1u) / X;
-  const uint32_t lo = x * share, hi = min(n, lo + share);
-  for (uint32_t i = lo + jb * 256u + threadIdx.x; i < hi; i += per_x * 256u) {
-    const ShadowJob j = q.jobs[i];
-    DenseRay r;
-    dense_ray_consts(r, w, sc.sun[0], sc.sun[1], sc.sun[2]);
-    r.wx = j.wx;
-    r.wy = j.wy;
-    r.wz = j.wz;
-    r.c = j.c;
-    r.steps = j.steps;
-    r.max_steps = sc.max_shadow;
-    FlatLookup lk;
-    DenseHit h;
-    dense_run_prefetch<FlatLookup, kPositive>(r, lk, w, h, sc.max_shadow);
-    write_pixel_shadow(v, sc, out, j, h);
-  }
-}
-
-__global__ void k_zero_count(uint32_t* count) {
-  if (threadIdx.x == 0u) count[0] = 0u;
 }
 
 template <bool kStats>
@@ -165,48 +123,28 @@ template <class L, bool kStats>
 static void launch_dense_shade(const DenseWorld& w, const ViewConsts& v, const ShadeConsts& sc,
                                const FrameOut& out, uint32_t log2_w, uint32_t blocks_x, dim3 grid,
                                hipStream_t stream) {
-  const ShadowQueue q{nullptr, nullptr};
   switch (shade_mode(sc.flags)) {
     case kShadeLambertGrad:
       hipLaunchKernelGGL((k_render_dense<L, kStats, kShadeLambertGrad>), grid, dim3(256), 0,
-                         stream, w, v, sc, out, log2_w, blocks_x, q);
+                         stream, w, v, sc, out, log2_w, blocks_x);
       break;
     case kShadeLambert:
       hipLaunchKernelGGL((k_render_dense<L, kStats, kShadeLambert>), grid, dim3(256), 0, stream, w,
-                         v, sc, out, log2_w, blocks_x, q);
+                         v, sc, out, log2_w, blocks_x);
       break;
     default:
       hipLaunchKernelGGL((k_render_dense<L, kStats, kShadeRef>), grid, dim3(256), 0, stream, w, v,
-                         sc, out, log2_w, blocks_x, q);
+                         sc, out, log2_w, blocks_x);
   }
-}
-
-bool render_defers_shadows(uint32_t traversal, const ShadeConsts& sc, const FrameOut& out) {
-  return traversal == VT_TRAVERSAL_FLAT && out.stats == nullptr &&
-         shade_mode(sc.flags) == kShadeRef && sc.max_shadow > kShadowPauseSteps;
 }
 
 hipError_t launch_render_dense(const DenseWorld& w, uint32_t traversal, const ViewConsts& v,
                                const ShadeConsts& sc, const FrameOut& out, TileShape tile,
-                               const ShadowQueue* queue, hipStream_t stream) {
+                               hipStream_t stream) {
   uint32_t blocks_x, nblocks;
   grid_shape(v, tile, blocks_x, nblocks);
   const dim3 grid(nblocks);
   const bool stats = out.stats != nullptr;
-  if (queue && render_defers_shadows(traversal, sc, out)) {
-    hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, stream, queue->count);
-    hipLaunchKernelGGL((k_render_dense<FlatLookup, false, kShadeRef, true>), grid, dim3(256), 0,
-                       stream, w, v, sc, out, tile.log2_w, blocks_x, *queue);
-    const uint32_t npix = v.W * v.H;
-    const uint32_t fblocks = std::min<uint32_t>(((npix + 255u) / 256u + 7u) & ~7u, 2048u);
-    if (sc.sun[0] >= 0.0f && sc.sun[1] >= 0.0f && sc.sun[2] >= 0.0f)
-      hipLaunchKernelGGL(k_shadow_finish<true>, dim3(fblocks), dim3(256), 0, stream, w, v, sc,
-                         out, *queue);
-    else
-      hipLaunchKernelGGL(k_shadow_finish<false>, dim3(fblocks), dim3(256), 0, stream, w, v, sc,
-                         out, *queue);
-    return hipGetLastError();
-  }
   if (traversal == VT_TRAVERSAL_BITMASK) {
     if (stats)
       launch_dense_shade<BitmaskLookup, true>(w, v, sc, out, tile.log2_w, blocks_x, grid, stream);

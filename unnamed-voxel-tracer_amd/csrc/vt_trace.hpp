@@ -689,16 +689,12 @@ __device__ __forceinline__ bool dense_skip_positive(float dx, float dy, float dz
 }
 
 // The flat lookup's traversal from r's block-level state: the empty-space
-// phase, then the exact loop.  With pause_at < r.max_steps the empty-space
-// phase stops at that step count instead; if it got there without meeting a
-// nonzero byte or leaving the grid, the ray is left unfinished: r holds the
-// state it continues from (within, code, steps) and the call returns false.
-// Otherwise the ray is finished into h and the call returns true.
+// phase, then the exact loop to the ray's end.
 // kPositive: the caller guarantees a direction positive on every axis (after
 // the zero substitution), and the empty-space phase is dense_skip_positive.
 template <class L, bool kPositive = false>
-__device__ __forceinline__ bool dense_run_prefetch(DenseRay& r, L& lk, const DenseWorld& w,
-                                                   DenseHit& h, int pause_at) {
+__device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const DenseWorld& w,
+                                                   DenseHit& h) {
   const uint32_t mx = in_vgpr(w.mx), my = in_vgpr(w.my), mz = in_vgpr(w.mz), flip = r.flip;
   const int max_steps = r.max_steps;
   DenseCell s0;
@@ -709,23 +705,12 @@ __device__ __forceinline__ bool dense_run_prefetch(DenseRay& r, L& lk, const Den
   const __amdgpu_buffer_rsrc_t vox =
       __builtin_amdgcn_make_buffer_rsrc((void*)w.vox, (short)0, (int)w.nvox, 0x00020000);
   int steps0 = r.steps;
-  const int limit = pause_at < max_steps ? pause_at : max_steps;
-  bool blocked;
   if constexpr (kPositive)
-    blocked = dense_skip_positive<VT_SKIP_STEPS>(r.dx, r.dy, r.dz, r.ix, r.iy, r.iz, mx, my, mz,
-                                                 s0, steps0, limit, vox);
+    dense_skip_positive<VT_SKIP_STEPS>(r.dx, r.dy, r.dz, r.ix, r.iy, r.iz, mx, my, mz, s0, steps0,
+                                       max_steps, vox);
   else
-    blocked = dense_skip_empty<VT_SKIP_STEPS>(r.Px, r.Py, r.Pz, r.Ax, r.Ay, r.Az, r.dx, r.dy,
-                                              r.dz, r.ix, r.iy, r.iz, mx, my, mz, flip, s0, steps0,
-                                              limit, vox);
-  if (!blocked && limit < max_steps) {
-    r.wx = s0.wx;
-    r.wy = s0.wy;
-    r.wz = s0.wz;
-    r.c = s0.c;
-    r.steps = steps0;
-    return false;
-  }
+    dense_skip_empty<VT_SKIP_STEPS>(r.Px, r.Py, r.Pz, r.Ax, r.Ay, r.Az, r.dx, r.dy, r.dz, r.ix,
+                                    r.iy, r.iz, mx, my, mz, flip, s0, steps0, max_steps, vox);
   float fwx, fwy, fwz;
   uint32_t fc, fM, fa, fv;
   int sF;
@@ -737,7 +722,7 @@ __device__ __forceinline__ bool dense_run_prefetch(DenseRay& r, L& lk, const Den
   if (sF >= max_steps || wrappedF) {  // the loop cap, then the bounds test
     h.hit = false;
     h.steps = sF;
-    return true;
+    return;
   }
   const uint32_t g = fc ^ flip;
   // faceId = 2 minIdx + 2 - rayPositivity[minIdx] (map.glsl:119-125)
@@ -756,14 +741,13 @@ __device__ __forceinline__ bool dense_run_prefetch(DenseRay& r, L& lk, const Den
   h.hy = (float)(compact1by2(g >> 1) << 3) + fwy;
   h.hz = (float)(compact1by2(g >> 2) << 3) + fwz;
   h.steps = sF + 1;
-  return true;
 }
 
 template <bool kStats, class L>
 __device__ __forceinline__ void dense_run(DenseRay& r, L& lk, const DenseWorld& w, DenseHit& h,
                                           uint32_t& l1f, uint32_t& l0f) {
   if constexpr (L::kPrefetch && !kStats) {
-    dense_run_prefetch(r, lk, w, h, r.max_steps);
+    dense_run_prefetch(r, lk, w, h);
     return;
   }
   const float Px = r.Px, Py = r.Py, Pz = r.Pz, Ax = r.Ax, Ay = r.Ay, Az = r.Az;
