@@ -351,6 +351,50 @@ __device__ __forceinline__ void write_pixel(const ViewConsts& v, const ShadeCons
   }
 }
 
+// The reference composite of a primary hit in two halves, so that only what
+// the second half needs stays live across the shadow traversal (the rest of
+// the primary hit would otherwise be spilled around the shadow loops).
+// First half: the stores that do not depend on the shadow ray.
+__device__ __forceinline__ void store_hit_primary(const ViewConsts& v, const ShadeConsts& sc,
+                                                  const FrameOut& out, size_t pix,
+                                                  const Primary& p) {
+  const bool unorm = (sc.flags & VT_FLAG_GBUF_UNORM8) != 0u;
+  float4 nrm, pos;
+  float t;
+  hit_normal_position(v, p, unorm, nrm, pos, t);
+  if (out.color) out.color[pix] = hit_color(p.data, unorm);
+  if (out.normal) out.normal[pix] = nrm;
+  if (out.position) out.position[pix] = pos;
+  if (out.hits) {
+    out.hits[3 * pix + 0] = p.voxel;
+    out.hits[3 * pix + 1] = __float_as_uint(t);
+  }
+}
+
+// Second half, once the shadow ray (shit, sh*, ssteps) is done: data = the
+// primary colour, info0 = its hit-record info word without the shadow bit.
+__device__ __forceinline__ void store_hit_shadow_ref(const ViewConsts& v, const ShadeConsts& sc,
+                                                     const FrameOut& out, uint32_t px,
+                                                     uint32_t py, uint32_t data, uint32_t info0,
+                                                     int psteps, bool shit, float shx,
+                                                     float shy, float shz, float sox, float soy,
+                                                     float soz, int ssteps, Counters& cnt) {
+  const size_t pix = (size_t)py * v.W + px;
+  const bool unorm = (sc.flags & VT_FLAG_GBUF_UNORM8) != 0u;
+  bool ent, occl;
+  const float4 ill = shadow_illum(sc, unorm, shit, shx, shy, shz, sox, soy, soz, ent, occl);
+  cnt.hits++;
+  cnt.occluded += occl ? 1u : 0u;
+  cnt.ent += ent ? 1u : 0u;
+  cnt.ssteps += (uint32_t)ssteps;
+  cnt.psteps += (uint32_t)psteps;
+  if (out.illum) out.illum[pix] = ill;
+  if (out.steps) out.steps[pix] = (uint32_t)psteps | ((uint32_t)ssteps << 16);
+  if (out.hits) out.hits[3 * pix + 2] = info0 | ((occl ? 1u : 0u) << 3);
+  if (out.shaded)
+    store_ref_composite(v, sc, out, pix, px, py, hit_color(data, unorm), ill, true);
+}
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
@@ -408,6 +452,9 @@ __device__ __forceinline__ void trace_pixel_dense(const DenseWorld& w, const Vie
     p.hz = h.hz;
     // ---- shadow ray (secondary.comp.glsl:25-42) ----
     shadow_origin(p, sc.flags, sox, soy, soz);
+    if constexpr (kShade == kShadeRef) store_hit_primary(v, sc, out, (size_t)py * v.W + px, p);
+    const uint32_t data = p.data, info0 = hit_info(p.face, p.sub, p.material, false);
+    const int psteps = p.steps;
     if (!dense_begin<kStats>(r, lk, w, sox, soy, soz, sc.sun[0], sc.sun[1], sc.sun[2],
                              sc.max_shadow, h, cnt.l1f, cnt.l0f)) {
       if constexpr (L::kPrefetch && !kStats) {
@@ -427,6 +474,11 @@ __device__ __forceinline__ void trace_pixel_dense(const DenseWorld& w, const Vie
       shz = h.hz;
     }
     ssteps = h.steps;
+    if constexpr (kShade == kShadeRef) {
+      store_hit_shadow_ref(v, sc, out, px, py, data, info0, psteps, shit, shx, shy, shz, sox, soy,
+                           soz, ssteps, cnt);
+      return;
+    }
   }
   write_pixel<kShade>(v, sc, out, px, py, p, shit, shx, shy, shz, sox, soy, soz, ssteps, cnt, &w,
                      gacc);
