@@ -107,6 +107,14 @@ def test_c0_sun_directions(ctx, c0, sun):
     o = O.render(ow, cam, W, H, 512, 512, sun)
     compare(g, o, label=f"c0 sun={sun}")
     w.close()
+    # the same world in the engine's layout (its shadow rays take the
+    # reference-layout kernel's loops)
+    import brickmap_util as BU
+    dim, chunks, pool, atlas = BU.embedding_a(grid, pal)
+    wb = vt.World.brickmap(ctx, dim, chunks, pool, atlas)
+    gb = gpu_render(ctx, wb, cam, W, H, 512, 512, sun=sun)
+    compare(gb, o, stats=False, label=f"c0 reference layout sun={sun}")
+    wb.close()
 
 
 def test_c0_reference_step_caps(ctx, c0):

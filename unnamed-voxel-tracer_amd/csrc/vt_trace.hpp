@@ -953,6 +953,10 @@ __device__ bool trace_general(const BrickmapWorld& w, float ox, float oy, float 
 // map or reaches the cap; the literal loop then resumes at that block's
 // lookup with the same grid / within / minIdx / step count.  Same hits, steps
 // and positions as trace_general.
+// kPositive: the caller guarantees a direction with no negative component (the
+// zero substitution below makes it positive), and the empty-space steps take
+// dense_skip_positive.
+template <bool kPositive = false>
 __device__ bool trace_brickmap_fast(const BrickmapWorld& w, float ox, float oy, float oz, float dx,
                                     float dy, float dz, int max_steps, GeneralHit& h) {
   if (dx == 0.0f) dx = 0.001f;
@@ -1025,9 +1029,13 @@ __device__ bool trace_brickmap_fast(const BrickmapWorld& w, float ox, float oy, 
       s0.c = morton3((uint32_t)g[0] >> 3, (uint32_t)g[1] >> 3, (uint32_t)g[2] >> 3) ^ flip;
       const uint32_t mx = in_vgpr(w.mx), my = in_vgpr(w.my), mz = in_vgpr(w.mz);
       int s0steps = step;
-      dense_skip_empty<VT_SKIP_STEPS>(P3[0], P3[1], P3[2], A3[0], A3[1], A3[2], d[0], d[1], d[2],
-                                      inv[0], inv[1], inv[2], mx, my, mz, flip, s0, s0steps,
-                                      max_steps, occ);
+      if constexpr (kPositive)
+        dense_skip_positive<VT_SKIP_STEPS>(d[0], d[1], d[2], inv[0], inv[1], inv[2], mx, my, mz,
+                                           s0, s0steps, max_steps, occ);
+      else
+        dense_skip_empty<VT_SKIP_STEPS>(P3[0], P3[1], P3[2], A3[0], A3[1], A3[2], d[0], d[1],
+                                        d[2], inv[0], inv[1], inv[2], mx, my, mz, flip, s0,
+                                        s0steps, max_steps, occ);
       float fw[3];
       uint32_t fc, fM, fa, fv;
       int sF;
