@@ -644,7 +644,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": measured_traffic(f"vt_render/{args.traversal}/512cube_1080p"),
+            "traffic": measured_traffic(f"k_render_dense/{args.traversal}/512cube_1080p"),
         },
         "cpu_baseline": cpu,
     }
