@@ -119,3 +119,48 @@ def test_edge_empty_reference_layout(ctx):
     compare(g, o, stats=False, label="empty reference layout")
     assert np.all(g["hits"][..., 0] == 0xFFFFFFFF)
     w.close()
+
+
+# Suns and cameras on exact diagonals: crossing times tie on two or three
+# axes at once, at every step, for rays that start on lattice points (all
+# shadow rays do).  These drive the tie rule (z > y > x), the within >= 8
+# correction on one or two axes at a time, and the empty-space loops' exact
+# replays (for a sun with no negative component, two axes at t <= 0 stop an
+# iteration).
+TIE_SUNS = [(0.5, 0.5, 0.5), (1.0, 1.0, 0.0), (0.6, 0.8, 0.0), (0.25, 0.5, 0.25),
+            (-0.5, 0.5, 0.5), (-1.0, -1.0, -1.0)]
+
+
+@pytest.mark.parametrize("sun", TIE_SUNS)
+@pytest.mark.parametrize("wname", ["checker32", "sparse64"])
+def test_edge_tied_suns(ctx, wname, sun):
+    grid = WORLDS[wname]
+    pal = S.palette()
+    n = grid.shape[0]
+    w = vt.World.dense(ctx, grid, pal)
+    ow = O.DenseWorld(grid, pal)
+    for cname, cam in cameras(n)[1:3] + cameras(n)[6:8]:
+        g = gpu_render(ctx, w, cam, 40, 33, 8 * n, 8 * n, sun=sun)
+        o = O.render(ow, cam, 40, 33, 8 * n, 8 * n, sun)
+        compare(g, o, label=f"{wname} {cname} sun={sun}")
+    w.close()
+
+
+def test_edge_diagonal_cameras(ctx):
+    """Primary rays along exact diagonals from lattice points."""
+    grid = WORLDS["sparse64"]
+    pal = S.palette()
+    w = vt.World.dense(ctx, grid, pal)
+    ow = O.DenseWorld(grid, pal)
+    cams = [
+        ("diag_ppp", vt.look_at((1.0, 1.0, 1.0), (63.0, 63.0, 63.0))),
+        ("diag_npn", vt.look_at((63.0, 1.0, 63.0), (1.0, 63.0, 1.0))),
+        ("diag_xy", vt.look_at((0.0, 0.0, 32.0), (64.0, 64.0, 32.0), up=(0.0, 0.0, 1.0))),
+        ("diag_mid", vt.look_at((32.0, 32.0, 32.0), (40.0, 40.0, 40.0))),
+    ]
+    for cname, cam in cams:
+        for (W, H) in [(1, 1), (41, 41)]:
+            g = gpu_render(ctx, w, cam, W, H, 512, 512, sun=(0.5, 0.5, 0.5))
+            o = O.render(ow, cam, W, H, 512, 512, (0.5, 0.5, 0.5))
+            compare(g, o, label=f"{cname} {W}x{H}")
+    w.close()
