@@ -223,6 +223,27 @@ int vt_world_set_voxels(vt_ctx* ctx, vt_world* world, const uint32_t* xyz,
  * material for dense worlds.  Host arrays; returns when values are written. */
 int vt_world_get_voxels(vt_ctx* ctx, const vt_world* world, const uint32_t* xyz, uint32_t n,
                         uint32_t* values, void* stream);
+/* The chunk table and block pool of a brickmap world as the CPU reads them
+ * through VoxelBrickmap's persistent mappings (voxel.zig:32-39,63-70):
+ * chunks[(dim/8)^3] and pool[0 .. *pool_words), 512 words per allocated
+ * block.  chunks or pool may be NULL; *pool_words is always set; a
+ * pool_capacity_words below the allocated words is an error
+ * (VT_ERR_INVALID_ARGUMENT).  Returns when the outputs are written.       */
+int vt_world_read_brickmap(vt_ctx* ctx, const vt_world* world, uint32_t* chunks,
+                           uint32_t* pool, size_t pool_capacity_words, size_t* pool_words,
+                           void* stream);
+
+/* The reference world generator, procgen(dim, &voxels, offset_x, offset_y)
+ * (procgen.zig:6-70, called by Game.init as procgen(512, &voxels, 0, 0),
+ * game.zig:96-97), on an empty brickmap world (no pool blocks yet; dim a
+ * multiple of 8, at least 16): a stone base layer, FastNoiseLite fBm
+ * terrain, decorations and trees, with the chunk table and pool equal word
+ * for word to what the generator's VoxelBrickmap.set calls leave.  A tree or
+ * terrain column that would write outside the map (small dims; the
+ * generator would index past its chunk table) is VT_ERR_INVALID_ARGUMENT and
+ * leaves the world unchanged.  Returns when the world is written.          */
+int vt_world_procgen(vt_ctx* ctx, vt_world* world, float offset_x, float offset_y,
+                     void* stream);
 
 /* terrain_edit.comp.glsl:11-24: the camera's centre ray (rayUV = 0) through
  * the AABB entry and traceMap with max_steps (64 in the shader).  place /

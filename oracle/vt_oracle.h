@@ -145,6 +145,20 @@ double or_lambert_grad64(const uint32_t* hits, uint32_t npix, const float* albed
                          const float sun[3], float ambient, const float* target_rgba,
                          double* dalbedo, double* dsun);
 
+/* ---- reference world generator (vt_procgen_oracle.c) ---- */
+/* FastNoiseLite gradient table (256 floats, pairs gx, gy). */
+void or_fnl_gradients(float out[256]);
+/* _fnlSingleSimplex2D on skewed coordinates. */
+float or_fnl_simplex2(int seed, float x, float y);
+/* FnlGenerator{.fractal_type = .fbm}.noise2 (procgen.zig:7,22). */
+float or_fnl_fbm2(float x, float y);
+/* procgen.zig:22-23: terrain height of column (x, z). */
+uint32_t or_procgen_height(uint32_t dim, float offset_x, float offset_y, uint32_t x, uint32_t z);
+/* procgen.zig:6-70 into an empty VoxelBrickmap(dim, 8); 0 = ok, 1 = pool
+ * too small, 2 = a write outside the map, 3 = bad dim. */
+int or_procgen(uint32_t dim, float offset_x, float offset_y, uint32_t* chunks, uint32_t* pool,
+               uint64_t pool_capacity_words, uint64_t* pool_words);
+
 #ifdef __cplusplus
 }
 #endif

@@ -81,8 +81,42 @@ def lib():
         L.or_lambert_grad64.restype = C.c_double
         L.or_lambert_grad64.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, f3, C.c_float,
                                         C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_fnl_gradients.argtypes = [C.c_void_p]
+        L.or_fnl_simplex2.restype = C.c_float
+        L.or_fnl_simplex2.argtypes = [C.c_int, C.c_float, C.c_float]
+        L.or_fnl_fbm2.restype = C.c_float
+        L.or_fnl_fbm2.argtypes = [C.c_float, C.c_float]
+        L.or_procgen_height.restype = C.c_uint32
+        L.or_procgen_height.argtypes = [C.c_uint32, C.c_float, C.c_float, C.c_uint32,
+                                        C.c_uint32]
+        L.or_procgen.restype = C.c_int
+        L.or_procgen.argtypes = [C.c_uint32, C.c_float, C.c_float, C.c_void_p, C.c_void_p,
+                                 C.c_uint64, C.POINTER(C.c_uint64)]
         _L = L
     return _L
+
+
+def procgen(dim, offset_x=0.0, offset_y=0.0):
+    """procgen.zig:6-70 on an empty VoxelBrickmap(dim, 8): (status, chunks, pool)."""
+    M = dim // 8
+    cap = M * M * 8
+    while True:
+        chunks = np.zeros(M ** 3, np.uint32)
+        pool = np.zeros(cap * 512, np.uint32)
+        words = C.c_uint64()
+        rc = lib().or_procgen(dim, offset_x, offset_y, chunks.ctypes.data, pool.ctypes.data,
+                              pool.size, C.byref(words))
+        if rc != 1:
+            return rc, chunks, pool[:words.value].copy()
+        cap *= 2
+
+
+def procgen_heights(dim, offset_x=0.0, offset_y=0.0, xs=None, zs=None):
+    """Column heights vh[x, z] (procgen.zig:22-23) of the given columns."""
+    xs = range(dim) if xs is None else xs
+    zs = range(dim) if zs is None else zs
+    f = lib().or_procgen_height
+    return np.array([[f(dim, offset_x, offset_y, x, z) for z in zs] for x in xs], np.uint32)
 
 
 def fvec(v):

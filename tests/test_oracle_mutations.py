@@ -9,10 +9,16 @@ import pytest
 
 import oracle_ffi as O
 import test_oracle_pinning as P
+import test_procgen_oracle as PG
 
-SRC = os.path.join(O.ORACLE_DIR, "vt_oracle.c")
+SOURCES = ("vt_oracle.c", "vt_procgen_oracle.c")
 
-# (name, original text, mutated text, pinning checks expected to fail)
+
+def _gen(check):
+    return lambda: check(PG.gen512_data())
+
+
+# (name, original text, mutated text, pinning checks expected to fail[, file])
 MUTATIONS = [
     ("anchor 0.999 -> 1.0", "<< s) * 0.999f;", "<< s) * 1.0f;",
      [P.test_trace_matches_amanatides_woo_with_drift, P.test_empty_world_steps_and_exit]),
@@ -47,18 +53,46 @@ MUTATIONS = [
     ("rgba8 byte order BGRA", "r |= (uint32_t)floorf(k * 255.0f + 0.5f) << (8 * i);",
      "r |= (uint32_t)floorf(k * 255.0f + 0.5f) << (8 * (i < 3 ? 2 - i : 3));",
      [P.test_present_rgba8_store]),
+    ("simplex far-corner constant sign", "const float K2 = (float)(-2 * (1 - 2 * G2)",
+     "const float K2 = (float)(2 * (1 - 2 * G2)",
+     [PG.test_simplex_zero_on_lattice, PG.test_simplex_equals_vertex_sum], "vt_procgen_oracle.c"),
+    ("simplex middle corners swapped", "n1 = (b * b) * (b * b) * fnl_grad(seed, ip, jp + FNL_PRIME_Y, x1, y1);",
+     "n1 = (b * b) * (b * b) * fnl_grad(seed, ip + FNL_PRIME_X, jp, x1, y1);",
+     [PG.test_simplex_equals_vertex_sum], "vt_procgen_oracle.c"),
+    ("simplex hash shift 13", "h ^= (uint32_t)((int32_t)h >> 15);",
+     "h ^= (uint32_t)((int32_t)h >> 13);", [PG.test_simplex_equals_vertex_sum],
+     "vt_procgen_oracle.c"),
+    ("fbm octave seed fixed", "float noise = or_fnl_simplex2(seed++, x, y);",
+     "float noise = or_fnl_simplex2(seed, x, y);",
+     [PG.test_fbm_equals_vertex_sum_and_is_bounded], "vt_procgen_oracle.c"),
+    ("fbm skew dropped", "  x += t;\n  y += t;\n  /* _fnlGenFractalFBM2D */",
+     "  /* _fnlGenFractalFBM2D */", [PG.test_fbm_equals_vertex_sum_and_is_bounded],
+     "vt_procgen_oracle.c"),
+    ("procgen closing draw dropped", "      (void)lcg(&s);\n    }\n  }",
+     "    }\n  }", [_gen(PG.test_lcg_replay_first_rows)], "vt_procgen_oracle.c"),
+    ("procgen top layer from the first draw", "else if (h == vh - 1 && h > 15)",
+     "else if (h == vh && h > 15)", [_gen(PG.test_column_layers)], "vt_procgen_oracle.c"),
+    ("procgen chunk allocation y-major",
+     "for (uint64_t x = 0; x < dim; x++)\n    for (uint64_t z = 0; z < dim; z++)\n      for (uint64_t y = 0; y < 16; y++)",
+     "for (uint64_t y = 0; y < 16; y++)\n    for (uint64_t x = 0; x < dim; x++)\n      for (uint64_t z = 0; z < dim; z++)",
+     [_gen(PG.test_base_layer_allocation_order)], "vt_procgen_oracle.c"),
 ]
 
 
-@pytest.mark.parametrize("name,orig,mut,checks", MUTATIONS, ids=[m[0] for m in MUTATIONS])
-def test_pinning_catches_planted_mistake(tmp_path, name, orig, mut, checks):
-    text = open(SRC).read()
-    assert text.count(orig) == 1, f"mutation anchor not found once: {orig}"
-    (tmp_path / "vt_oracle.c").write_text(text.replace(orig, mut))
-    (tmp_path / "vt_oracle.h").write_text(open(os.path.join(O.ORACLE_DIR, "vt_oracle.h")).read())
+@pytest.mark.parametrize("mutation", MUTATIONS, ids=[m[0] for m in MUTATIONS])
+def test_pinning_catches_planted_mistake(tmp_path, mutation):
+    name, orig, mut, checks = mutation[:4]
+    target = mutation[4] if len(mutation) > 4 else "vt_oracle.c"
+    for f in SOURCES + ("vt_oracle.h",):
+        text = open(os.path.join(O.ORACLE_DIR, f)).read()
+        if f == target:
+            assert text.count(orig) == 1, f"mutation anchor not found once: {orig}"
+            text = text.replace(orig, mut)
+        (tmp_path / f).write_text(text)
     so = tmp_path / "liboracle.so"
     subprocess.run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-ffp-contract=off", "-o",
-                    str(so), str(tmp_path / "vt_oracle.c"), "-lm", "-lpthread"], check=True)
+                    str(so), *[str(tmp_path / f) for f in SOURCES], "-lm", "-lpthread"],
+                   check=True)
     saved_dir, saved_lib = O.ORACLE_DIR, O._L
     try:
         O.ORACLE_DIR, O._L = str(tmp_path), None

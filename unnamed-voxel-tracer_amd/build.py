@@ -18,7 +18,7 @@ LIB = os.path.join(PKG, "lib")
 CSRC = os.path.join(PKG, "csrc")
 
 HIP_SOURCES = ["vt_api.hip", "vt_render.hip", "vt_world.hip", "vt_batch.hip", "vt_lambert.hip",
-               "vt_present.hip", "vt_vox.hip", "vt_edit.hip"]
+               "vt_present.hip", "vt_vox.hip", "vt_edit.hip", "vt_procgen.hip"]
 
 # -ffp-contract=off: every binary32 multiply and add rounds separately (the
 #   traversal must reproduce the CPU restatement bit for bit).

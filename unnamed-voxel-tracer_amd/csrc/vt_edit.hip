@@ -159,6 +159,33 @@ __global__ void k_pick_brickmap(const BrickmapWorld w, ViewConsts v, int max_ste
   *out = pick_result(g.data != 0u, g.data, g.face, g.hx, g.hy, g.hz, g.steps);
 }
 
+// Grows the block pool to at least `blocks` blocks (doubling), keeping the
+// contents and zeroing the new blocks.
+int grow_pool(vt_world* w, uint64_t blocks, hipStream_t s, const char* who) {
+  const uint64_t cap = w->pool_capacity / 512u;
+  if (blocks <= cap) return VT_OK;
+  uint64_t ncap = cap ? cap : 1;
+  while (ncap < blocks) ncap *= 2;
+  uint32_t* np = nullptr;
+  hipError_t e = hipMalloc((void**)&np, ncap * 512u * 4u);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return hip_fail(e, who);
+  }
+  e = hipMemcpyAsync(np, w->pool, cap * 512u * 4u, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemsetAsync(np + cap * 512u, 0, (ncap - cap) * 512u * 4u, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    dev_free(np);
+    return hip_fail(e, who);
+  }
+  dev_free(w->pool);
+  w->pool = np;
+  w->bytes += (ncap - cap) * 512u * 4u;
+  w->pool_capacity = ncap * 512u;
+  return VT_OK;
+}
+
 }  // namespace vt
 
 namespace {
@@ -208,33 +235,6 @@ int check_coords(const vt_world* w, const uint32_t* xyz, uint32_t n, const char*
   return VT_OK;
 }
 
-// Grows the block pool to at least `blocks` blocks (doubling), keeping the
-// contents and zeroing the new blocks.
-int grow_pool(vt_world* w, uint64_t blocks, hipStream_t s) {
-  const uint64_t cap = w->pool_capacity / 512u;
-  if (blocks <= cap) return VT_OK;
-  uint64_t ncap = cap ? cap : 1;
-  while (ncap < blocks) ncap *= 2;
-  uint32_t* np = nullptr;
-  hipError_t e = hipMalloc((void**)&np, ncap * 512u * 4u);
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
-    return hip_fail(e, "vt_world_set_voxels: pool growth");
-  }
-  e = hipMemcpyAsync(np, w->pool, cap * 512u * 4u, hipMemcpyDeviceToDevice, s);
-  if (e == hipSuccess) e = hipMemsetAsync(np + cap * 512u, 0, (ncap - cap) * 512u * 4u, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e != hipSuccess) {
-    vt::dev_free(np);
-    return hip_fail(e, "vt_world_set_voxels: pool growth");
-  }
-  vt::dev_free(w->pool);
-  w->pool = np;
-  w->bytes += (ncap - cap) * 512u * 4u;
-  w->pool_capacity = ncap * 512u;
-  return VT_OK;
-}
-
 int set_brickmap(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint32_t n,
                  hipStream_t s) {
   const uint32_t M = w->dim / 8u;
@@ -264,7 +264,7 @@ int set_brickmap(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint3
       val[it->second] = values[i];
     }
   }
-  int st = grow_pool(w, used, s);
+  int st = vt::grow_pool(w, used, s, "vt_world_set_voxels: pool growth");
   if (st != VT_OK) return st;  // (the mirror keeps the allocations: same as a
                                //  reference resize panic, the world is unusable)
   std::vector<uint64_t> caddr;
@@ -396,6 +396,30 @@ int vt_world_get_voxels(vt_ctx* ctx, const vt_world* world, const uint32_t* xyz,
                        vt::brickmap_view(world), d_xyz, n, d_out);
   VT_HIP(hipGetLastError());
   VT_HIP(hipMemcpyAsync(values, d_out, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+  VT_HIP(hipStreamSynchronize(s));
+  return VT_OK;
+}
+
+int vt_world_read_brickmap(vt_ctx* ctx, const vt_world* world, uint32_t* chunks,
+                           uint32_t* pool, size_t pool_capacity_words, size_t* pool_words,
+                           void* stream) {
+  if (!ctx || !world || !pool_words)
+    return fail(VT_ERR_NULL_POINTER, "vt_world_read_brickmap: null argument");
+  if (world->ctx != ctx) return fail(VT_ERR_WRONG_CONTEXT, "vt_world_read_brickmap");
+  if (world->kind != WorldKind::Brickmap)
+    return fail(VT_ERR_INVALID_ARGUMENT, "vt_world_read_brickmap: needs a brickmap world");
+  *pool_words = world->pool_words;
+  if (pool && pool_capacity_words < world->pool_words)
+    return fail(VT_ERR_INVALID_ARGUMENT,
+                "vt_world_read_brickmap: pool holds %zu words, the output room is %zu",
+                world->pool_words, pool_capacity_words);
+  VT_HIP(hipSetDevice(ctx->device));
+  const hipStream_t s = (hipStream_t)stream;
+  const size_t M = world->dim / 8u;
+  if (chunks)
+    VT_HIP(hipMemcpyAsync(chunks, world->chunks, M * M * M * 4, hipMemcpyDeviceToHost, s));
+  if (pool && world->pool_words)
+    VT_HIP(hipMemcpyAsync(pool, world->pool, world->pool_words * 4, hipMemcpyDeviceToHost, s));
   VT_HIP(hipStreamSynchronize(s));
   return VT_OK;
 }

@@ -108,6 +108,10 @@ hipError_t launch_render_brickmap(const BrickmapWorld& w, const ViewConsts& v,
 // World construction kernels.
 // occ[morton(b)] = (map_getVoxel(b) != 0) for every block b of a brickmap world.
 hipError_t launch_build_occupancy(const BrickmapWorld& w, uint8_t* occ, hipStream_t stream);
+// Grows a brickmap world's block pool to at least `blocks` blocks by
+// doubling (GpuBlockAllocator.alloc, gpu_block_allocator.zig:20-30), keeping
+// the contents and zeroing the new blocks; `who` names the caller in errors.
+int grow_pool(vt_world* w, uint64_t blocks, hipStream_t s, const char* who);
 hipError_t launch_build_dense(const uint8_t* linear, uint32_t dim, uint8_t* vox, uint64_t* l0,
                               uint64_t* l1, hipStream_t stream);
 // Exclusive prefix sum of popcount(l0[i]) into prefix[0..nwords); scratch

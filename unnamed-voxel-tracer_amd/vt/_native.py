@@ -129,6 +129,8 @@ def lib() -> C.CDLL:
         "vt_atlas_load_vox": (i32, [vp, C.c_size_t, vp, C.POINTER(u32)]),
         "vt_world_set_voxels": (i32, [vp, vp, vp, vp, u32, vp]),
         "vt_world_get_voxels": (i32, [vp, vp, vp, u32, vp, vp]),
+        "vt_world_read_brickmap": (i32, [vp, vp, vp, vp, C.c_size_t, C.POINTER(C.c_size_t), vp]),
+        "vt_world_procgen": (i32, [vp, vp, C.c_float, C.c_float, vp]),
         "vt_pick": (i32, [vp, vp, C.POINTER(Camera), i32, C.POINTER(PickResult), vp]),
         "vt_world_build_compact_index": (i32, [vp, vp, C.POINTER(C.c_uint64)]),
         "vt_lambert_shade": (i32, [vp, vp, vp, u32, C.POINTER(RenderParams), vp, vp]),

@@ -102,6 +102,27 @@ class World:
                 "vt_world_get_voxels")
         return out
 
+    def read_brickmap(self, stream=None):
+        """(chunks, pool) of a brickmap world, as VoxelBrickmap's mappings hold them."""
+        M = self.dim // 8
+        words = C.c_size_t()
+        N.check(N.lib().vt_world_read_brickmap(self.ctx.h, self.h, None, None, 0, C.byref(words),
+                                               C.c_void_p(_stream(stream))),
+                "vt_world_read_brickmap")
+        chunks = np.zeros(M ** 3, np.uint32)
+        pool = np.zeros(words.value, np.uint32)
+        N.check(N.lib().vt_world_read_brickmap(self.ctx.h, self.h, chunks.ctypes.data,
+                                               pool.ctypes.data if pool.size else None,
+                                               pool.size, C.byref(words),
+                                               C.c_void_p(_stream(stream))),
+                "vt_world_read_brickmap")
+        return chunks, pool
+
+    def procgen(self, offset_x: float = 0.0, offset_y: float = 0.0, stream=None):
+        """procgen.zig:6-70 on this (empty) brickmap world; see vt_world_procgen."""
+        N.check(N.lib().vt_world_procgen(self.ctx.h, self.h, offset_x, offset_y,
+                                         C.c_void_p(_stream(stream))), "vt_world_procgen")
+
     def pick(self, cam: N.Camera, max_steps: int = 64, stream=None) -> N.PickResult:
         """Centre-ray pick of terrain_edit.comp.glsl (64 steps there)."""
         r = N.PickResult()
