@@ -511,6 +511,78 @@ def measured_traffic(key):
         return None
 
 
+def bench_procgen(args):
+    """The engine's world generator as Game.init runs it, procgen(512,
+    &voxels, 0, 0) (game.zig:96-97), into a fresh empty 512^3 brickmap world
+    per step (worlds created before the timed region).  vt_world_procgen
+    returns when the world is written, so each call is timed on the host
+    clock; it includes the device heights, the host walk over the 262,144
+    columns, the uploads, the column/tree kernels and the occupancy rebuild.
+    cpu_baseline: the oracle's literal restatement (one thread) of the same
+    world, whose chunk table and pool must equal the device's."""
+    import numpy as np
+    import torch
+
+    import vt
+
+    ctx = vt.Context(0)
+    atlas = np.zeros(256 ** 3, np.uint32)  # procgen writes blocks only
+    worlds = [vt.World.empty_brickmap(ctx, 512, atlas) for _ in range(args.warmup + args.steps)]
+    torch.cuda.synchronize()
+    for w in worlds[:args.warmup]:
+        w.procgen()
+    dts = []
+    for w in worlds[args.warmup:]:
+        t0 = time.perf_counter()
+        w.procgen()
+        dts.append(time.perf_counter() - t0)
+    ms = 1e3 * sum(dts) / len(dts)
+    chunks, pool = worlds[-1].read_brickmap()
+    cpu = None
+    equal = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_ffi as O
+        reps = 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            rc, oc, op = O.procgen(512)
+        cdt = (time.perf_counter() - t0) / reps
+        equal = bool(rc == 0 and np.array_equal(oc, chunks) and np.array_equal(op, pool))
+        cpu = {"value": round(1e3 * cdt, 3), "unit": "ms per 512^3 world", "cores": 1,
+               "kind": "port",
+               "sample": f"the same world generated {reps} times by the oracle's literal "
+                         f"restatement, 1 thread, {cdt * reps:.2f} s wall"}
+    line = {
+        "metric": "ms per 512^3 world, reference procgen (FastNoiseLite fBm terrain, "
+                  "decorations, trees) into the engine's chunk/pool layout",
+        "value": round(ms, 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+u32",
+        "data": "synthetic",
+        "config": {
+            "workload": "procgen(512, 0, 0) into an empty VoxelBrickmap(512, 8): 262,144 "
+                        f"columns, {len(pool) // 512} pool blocks",
+            "min_ms": round(1e3 * min(dts), 4),
+            "columns_per_s": round(512 * 512 / (ms * 1e-3)),
+            "chunks_and_pool_equal_oracle": equal,
+        },
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    for w in worlds:
+        w.close()
+    ctx.close()
+
+
 def bench_copy(args):
     """Calibration for the HBM counters: one plain device-to-device copy of
     1 GiB (torch's copy kernel) per step -- 1 GiB read + 1 GiB written."""
@@ -535,12 +607,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--traversal", choices=["flat", "bitmask"], default="flat")
     ap.add_argument("--workload",
-                    choices=["frame", "batched", "diff", "diff_batched", "brickmap", "copy"],
+                    choices=["frame", "batched", "diff", "diff_batched", "brickmap", "procgen",
+                             "copy"],
                     default="frame",
                     help="frame: 512^3, one 1080p view (the headline); batched: 1024^3, 64 "
                          "uneven cameras in one persistent graph launch; diff: 256^3, 16 "
                          "views fwd+bwd; diff_batched: 1024^3 compact albedo, 48 views fwd+bwd; "
                          "brickmap: the headline world in the engine's own layout; "
+                         "procgen: the engine's 512^3 world generator; "
                          "copy: 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
@@ -559,6 +633,8 @@ def main():
         return bench_copy(args)
     if args.workload == "brickmap":
         return bench_brickmap(args)
+    if args.workload == "procgen":
+        return bench_procgen(args)
 
     import numpy as np
     import torch

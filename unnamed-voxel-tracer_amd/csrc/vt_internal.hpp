@@ -106,8 +106,10 @@ hipError_t launch_render_brickmap(const BrickmapWorld& w, const ViewConsts& v,
                                   hipStream_t stream);
 
 // World construction kernels.
-// occ[morton(b)] = (map_getVoxel(b) != 0) for every block b of a brickmap world.
-hipError_t launch_build_occupancy(const BrickmapWorld& w, uint8_t* occ, hipStream_t stream);
+// occ[morton(b)] = (map_getVoxel(b) != 0) for every block b of a brickmap
+// world with y < y_end (0: all layers).
+hipError_t launch_build_occupancy(const BrickmapWorld& w, uint8_t* occ, hipStream_t stream,
+                                  uint32_t y_end = 0);
 // Grows a brickmap world's block pool to at least `blocks` blocks by
 // doubling (GpuBlockAllocator.alloc, gpu_block_allocator.zig:20-30), keeping
 // the contents and zeroing the new blocks; `who` names the caller in errors.

@@ -34,6 +34,7 @@
 //      terrain of such a later column overwrote whatever tree voxels lay
 //      below its height, and nothing else writes above it.
 //   5. The block-occupancy bytes of the fast traversal are rebuilt.
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <unordered_map>
@@ -196,12 +197,17 @@ __global__ __launch_bounds__(64) void k_procgen_columns(uint32_t dim,
   }
 }
 
+struct TreeVoxel {
+  uint64_t addr;  // pool word
+  uint32_t value;
+  uint32_t pad;
+};
+
 __global__ __launch_bounds__(256) void k_procgen_trees(uint32_t* __restrict__ pool,
-                                                      const uint64_t* __restrict__ addr,
-                                                      const uint32_t* __restrict__ val,
+                                                      const TreeVoxel* __restrict__ tv,
                                                       uint32_t n) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i < n) pool[addr[i]] = val[i];
+  if (i < n) pool[tv[i].addr] = tv[i].value;
 }
 
 // The host walk (steps 2 and 4 of the header).
@@ -209,8 +215,10 @@ struct Walk {
   uint32_t dim, M;
   std::vector<uint32_t> mirror;  // chunk table being allocated
   uint64_t used = 0;             // blocks allocated
-  std::vector<uint32_t> state, decor;
+  std::vector<uint32_t> cols;  // [col] LCG state at the column, [ncol + col] decoration
   std::unordered_map<uint64_t, uint32_t> trees;  // voxel x + dim (y + dim z) -> word
+  std::vector<uint8_t> tree_col;                 // column x * dim + z holds a tree voxel
+  uint32_t y_end = 16;                           // layers the generator wrote
   std::vector<uint32_t> jump_a, jump_c;          // n draws: s -> jump_a[n] s + jump_c[n]
   uint32_t bad_x = 0, bad_z = 0;
 
@@ -222,6 +230,8 @@ struct Walk {
   void tree_set(uint32_t x, uint32_t y, uint32_t z, uint32_t v) {
     touch(x, y, z);
     trees[key(x, y, z)] = v;
+    tree_col[(size_t)x * dim + z] = 1;
+    if (y + 1u > y_end) y_end = y + 1u;
   }
   static uint32_t next(uint32_t& s) {
     s = s * kLcgMul + kLcgAdd;
@@ -259,8 +269,9 @@ struct Walk {
         touch(cx * 8u, 0u, cz * 8u);
         touch(cx * 8u, 8u, cz * 8u);
       }
-    state.assign((size_t)dim * dim, 0u);
-    decor.assign((size_t)dim * dim, 0u);
+    const size_t ncol = (size_t)dim * dim;
+    cols.assign(2 * ncol, 0u);
+    tree_col.assign((size_t)dim * dim, 0u);
     uint32_t s = kLcgSeed;
     for (uint32_t x = 0; x < dim; x++) {
       for (uint32_t z = 0; z < dim; z++) {
@@ -271,19 +282,20 @@ struct Walk {
           bad_z = z;
           return false;
         }
-        state[col] = s;
+        cols[col] = s;
         if (h <= 16u) {  // two draws per layer, then the closing draw
           s = jump_a[2u * h + 1u] * s + jump_c[2u * h + 1u];
           continue;
         }
         for (uint32_t y = 16u; y < h; y += 8u) touch(x, y, z);
+        if (h + 1u > y_end) y_end = h + 1u;  // (with the decoration)
         s = jump_a[h + 17u] * s + jump_c[h + 17u];
-        if (trees.count(key(x, h, z))) continue;  // procgen.zig:37-38
+        if (tree_col[col] && trees.count(key(x, h, z))) continue;  // procgen.zig:37-38
         uint32_t d = 0;
         if (next(s) % 5u == 0) d = 7u + next(s) % 5u;
         if (next(s) % 71u == 0) d = (7u + 5u) | kSolid;
         if (d) touch(x, h, z);
-        decor[col] = d;
+        cols[ncol + col] = d;
         if (next(s) % 420u == 0 && x < 500u && z < 500u && x > 5u && z > 5u)
           if (!place_tree(s, x, h, z)) return false;
         next(s);
@@ -330,8 +342,7 @@ int vt_world_procgen(vt_ctx* ctx, vt_world* world, float offset_x, float offset_
       return e;
     }
   } bufs;
-  uint32_t *d_vh, *d_state, *d_decor, *d_tval = nullptr;
-  uint64_t* d_taddr = nullptr;
+  uint32_t* d_vh;
   VT_HIP(bufs.alloc((void**)&d_vh, ncol * 4));
   hipLaunchKernelGGL(vt::k_procgen_heights, dim3((uint32_t)((ncol + 255) / 256)), dim3(256), 0, s,
                      dim, offset_x, offset_y, d_vh);
@@ -349,42 +360,41 @@ int vt_world_procgen(vt_ctx* ctx, vt_world* world, float offset_x, float offset_
                 "vt_world_procgen: column (%u, %u) writes outside the %u^3 map (its terrain, "
                 "or its tree: place_tree writes up to x+2, z+2 and 9 layers up)",
                 walk.bad_x, walk.bad_z, dim);
-  std::vector<uint64_t> taddr;
-  std::vector<uint32_t> tval;
-  taddr.reserve(walk.trees.size());
-  tval.reserve(walk.trees.size());
+  std::vector<vt::TreeVoxel> tv;
+  tv.reserve(walk.trees.size());
   for (const auto& kv : walk.trees) {
     const uint64_t x = kv.first % dim, y = kv.first / dim % dim, z = kv.first / dim / dim;
     if (y < vh[x * dim + z]) continue;  // under a later column's terrain
     const uint32_t e = walk.mirror[x / 8u + M * (y / 8u + M * (z / 8u))];
-    taddr.push_back((uint64_t)(e - 1u) * 512u + x % 8u + 8u * (y % 8u + 8u * (z % 8u)));
-    tval.push_back(kv.second);
+    tv.push_back({(uint64_t)(e - 1u) * 512u + x % 8u + 8u * (y % 8u + 8u * (z % 8u)), kv.second,
+                  0u});
   }
 
   int st = vt::grow_pool(world, walk.used, s, "vt_world_procgen: pool growth");
   if (st != VT_OK) return st;
-  VT_HIP(bufs.alloc((void**)&d_state, ncol * 4));
-  VT_HIP(bufs.alloc((void**)&d_decor, ncol * 4));
-  VT_HIP(bufs.alloc((void**)&d_taddr, taddr.size() * 8));
-  VT_HIP(bufs.alloc((void**)&d_tval, tval.size() * 4));
+  // one staging allocation: column states and decorations, then the tree voxels
+  uint32_t* d_cols;
+  VT_HIP(bufs.alloc((void**)&d_cols, 2 * ncol * 4 + tv.size() * sizeof(vt::TreeVoxel)));
+  vt::TreeVoxel* d_tv = (vt::TreeVoxel*)(d_cols + 2 * ncol);
   VT_HIP(hipMemcpyAsync(world->chunks, walk.mirror.data(), nchunks * 4, hipMemcpyHostToDevice, s));
-  VT_HIP(hipMemcpyAsync(d_state, walk.state.data(), ncol * 4, hipMemcpyHostToDevice, s));
-  VT_HIP(hipMemcpyAsync(d_decor, walk.decor.data(), ncol * 4, hipMemcpyHostToDevice, s));
-  if (!taddr.empty()) {
-    VT_HIP(hipMemcpyAsync(d_taddr, taddr.data(), taddr.size() * 8, hipMemcpyHostToDevice, s));
-    VT_HIP(hipMemcpyAsync(d_tval, tval.data(), tval.size() * 4, hipMemcpyHostToDevice, s));
-  }
+  VT_HIP(hipMemcpyAsync(d_cols, walk.cols.data(), 2 * ncol * 4, hipMemcpyHostToDevice, s));
+  if (!tv.empty())
+    VT_HIP(hipMemcpyAsync(d_tv, tv.data(), tv.size() * sizeof(vt::TreeVoxel),
+                          hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(vt::k_procgen_columns, dim3((uint32_t)(M * M)), dim3(64), 0, s, dim,
-                     world->chunks, world->pool, d_vh, d_state, d_decor);
+                     world->chunks, world->pool, d_vh, d_cols, d_cols + ncol);
   VT_HIP(hipGetLastError());
-  if (!taddr.empty()) {
-    hipLaunchKernelGGL(vt::k_procgen_trees, dim3((uint32_t)((taddr.size() + 255) / 256)),
-                       dim3(256), 0, s, world->pool, d_taddr, d_tval, (uint32_t)taddr.size());
+  if (!tv.empty()) {
+    hipLaunchKernelGGL(vt::k_procgen_trees, dim3((uint32_t)((tv.size() + 255) / 256)), dim3(256),
+                       0, s, world->pool, d_tv, (uint32_t)tv.size());
     VT_HIP(hipGetLastError());
   }
   std::memcpy(world->chunk_mirror, walk.mirror.data(), nchunks * 4);
   world->pool_words = walk.used * 512u;
-  if (world->occ) VT_HIP(vt::launch_build_occupancy(vt::brickmap_view(world), world->occ, s));
+  // the empty map's occupancy bytes were zero; only the written layers change
+  if (world->occ)
+    VT_HIP(vt::launch_build_occupancy(vt::brickmap_view(world), world->occ, s,
+                                      std::min(dim, (walk.y_end + 7u) & ~7u)));
   VT_HIP(hipStreamSynchronize(s));
   return VT_OK;
 }

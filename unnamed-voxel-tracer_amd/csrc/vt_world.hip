@@ -167,20 +167,24 @@ hipError_t launch_build_dense(const uint8_t* linear, uint32_t dim, uint8_t* vox,
   return hipGetLastError();
 }
 
-// One thread per block of a brickmap world (linear order x + dim*(y + dim*z)).
-__global__ __launch_bounds__(256) void k_build_occupancy(BrickmapWorld w, uint8_t* __restrict__ occ) {
+// One thread per block of the layers y < y_end of a brickmap world (linear
+// order x + dim*(y + y_end*z)).
+__global__ __launch_bounds__(256) void k_build_occupancy(BrickmapWorld w, uint8_t* __restrict__ occ,
+                                                         uint32_t y_end) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t n = (uint64_t)w.dim * w.dim * w.dim;
+  const uint64_t n = (uint64_t)w.dim * y_end * w.dim;
   if (i >= n) return;
-  const uint32_t x = (uint32_t)(i % w.dim), y = (uint32_t)((i / w.dim) % w.dim),
-                 z = (uint32_t)(i / ((uint64_t)w.dim * w.dim));
+  const uint32_t x = (uint32_t)(i % w.dim), y = (uint32_t)((i / w.dim) % y_end),
+                 z = (uint32_t)(i / ((uint64_t)w.dim * y_end));
   occ[morton3(x, y, z)] = bm_block(w, (int)x, (int)y, (int)z) != 0u ? 1 : 0;
 }
 
-hipError_t launch_build_occupancy(const BrickmapWorld& w, uint8_t* occ, hipStream_t stream) {
-  const uint64_t n = (uint64_t)w.dim * w.dim * w.dim;
+hipError_t launch_build_occupancy(const BrickmapWorld& w, uint8_t* occ, hipStream_t stream,
+                                  uint32_t y_end) {
+  if (y_end == 0 || y_end > w.dim) y_end = w.dim;
+  const uint64_t n = (uint64_t)w.dim * y_end * w.dim;
   hipLaunchKernelGGL(k_build_occupancy, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream, w,
-                     occ);
+                     occ, y_end);
   return hipGetLastError();
 }
 
