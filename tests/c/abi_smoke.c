@@ -1,6 +1,7 @@
 /* The C-ABI from C, as an engine's render shim would use libvt.so: context,
  * a dense world with one voxel, a render, the centre pick, an edit that
- * removes the voxel, error statuses.
+ * removes the voxel, error statuses; then the engine's world generator on an
+ * empty brickmap world and a read-back of its chunk table.
  * Build: gcc -std=c11 -D__HIP_PLATFORM_AMD__ -I include -I /opt/rocm/include
  *        tests/c/abi_smoke.c -L <lib dir> -lvt -L /opt/rocm/lib -lamdhip64 -lm
  * Prints "ABI SMOKE OK" and exits 0 when every check holds. */
@@ -8,6 +9,7 @@
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "vt.h"
@@ -135,6 +137,40 @@ int main(void) {
   st = vt_world_create_dense_u8(ctx, grid, 24, pal, &bad);
   CHECK(st == VT_ERR_INVALID_ARGUMENT && bad == NULL, "dim 24 -> %d", st);
   CHECK(strlen(vt_status_string(VT_ERR_DEVICE)) > 0, "status string");
+
+  /* Game.init's world in the engine's layout: empty VoxelBrickmap, procgen */
+  {
+    const uint32_t D = 64, M = D / 8;
+    static uint32_t zero_chunks[8 * 8 * 8];
+    uint32_t* atlas = calloc(256u * 256u * 256u, 4);
+    CHECK(atlas != NULL, "atlas");
+    vt_world* bm = NULL;
+    st = vt_world_create_brickmap(ctx, D, zero_chunks, NULL, 0, atlas, &bm);
+    CHECK(st == VT_OK && bm, "vt_world_create_brickmap: %s", vt_status_string(st));
+    st = vt_world_procgen(ctx, bm, 0.0f, 0.0f, NULL);
+    CHECK(st == VT_OK, "vt_world_procgen: %s", vt_status_string(st));
+    size_t words = 0;
+    static uint32_t chunks[8 * 8 * 8];
+    st = vt_world_read_brickmap(ctx, bm, chunks, NULL, 0, &words, NULL);
+    /* the base layer allocates chunks (cx, 0, cz), (cx, 1, cz) first, in x-major order */
+    CHECK(st == VT_OK && words >= 2u * M * M * 512u && chunks[0] == 1 && chunks[M] == 2 &&
+              chunks[M * M] == 3,
+          "vt_world_read_brickmap: %zu words, chunks %u %u %u", words, chunks[0], chunks[M],
+          chunks[M * M]);
+    const uint32_t base[3] = {3, 15, 40}, top[3] = {3, 16, 40};
+    uint32_t v[2];
+    st = vt_world_get_voxels(ctx, bm, base, 1, v, NULL);
+    st |= vt_world_get_voxels(ctx, bm, top, 1, v + 1, NULL);
+    /* y = 15: stone (13) or the sand-like 25..27; a 64^3 map's terrain stays below y = 16 */
+    CHECK(st == VT_OK && v[0] >> 28 == 1u &&
+              ((v[0] & 0xFFFFFFFu) == 13u || ((v[0] & 0xFFFFFFFu) >= 25u && (v[0] & 0xFFFFFFFu) <= 27u)) &&
+              v[1] == 0u,
+          "procgen voxels %08x %08x", v[0], v[1]);
+    st = vt_world_procgen(ctx, bm, 0.0f, 0.0f, NULL);
+    CHECK(st == VT_ERR_INVALID_ARGUMENT, "procgen on a non-empty world -> %d", st);
+    CHECK(vt_world_destroy(bm) == VT_OK, "vt_world_destroy");
+    free(atlas);
+  }
 
   CHECK(vt_world_destroy(world) == VT_OK, "vt_world_destroy");
   CHECK(vt_destroy(ctx) == VT_OK, "vt_destroy");

@@ -9,7 +9,8 @@
 * The generated world (with the engine's 29-slot model atlas) renders
   exactly like a world created from the oracle's buffers, and like the
   oracle's render on a band of rows: the occupancy bytes of the fast
-  traversal were rebuilt.
+  traversal were rebuilt.  The engine's own first frame (Game.init's camera)
+  and a turned view, 1920x1080, equal the oracle's full frames.
 * Argument errors: dense worlds, non-empty worlds, NaN offsets.
 """
 import numpy as np
@@ -92,6 +93,42 @@ def test_procgen_world_renders(ctx, atlas):
         assert (fh[..., 0] != 0xFFFFFFFF).mean() > 0.3
     w.close()
     ref.close()
+
+
+def test_procgen_engine_first_frames(ctx, atlas):
+    """The engine's own first frame: Game.init's world and camera (player at
+    (256, 22, 256), camera 3 blocks up, identity rotation, fov pi/2,
+    camera.zig:5-10, game.zig:40,212), then a turned view; full 1920x1080
+    frames against the oracle."""
+    import math
+
+    import torch
+    from vt import _native as N
+    rc, chunks, pool = O.procgen(512)
+    assert rc == 0
+    w = vt.World.empty_brickmap(ctx, 512, atlas)
+    w.procgen()
+    ow = O.BrickmapWorld(512, chunks, pool, atlas)
+    start = N.Camera()
+    start.position[:] = [256.0, 25.0, 256.0, 0.0]
+    start.view[:] = [1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0]
+    start.fov = float(np.float32(math.pi / 2.0))
+    turned = vcam.look_at((256.0, 25.0, 256.0), (180.0, 12.0, 330.0), 90.0)
+    W, H = 1920, 1080
+    for k, cam in enumerate((start, turned)):
+        prm = vt.params(W, H, 2048, 2048, S.SUN)
+        f = vt.Frame.alloc(W, H, steps=True)
+        log(f"engine frame {k}")
+        vt.render(ctx, w, cam, f, prm)
+        torch.cuda.synchronize()
+        o = O.render(ow, cam, W, H, 2048, 2048, S.SUN, gbuffer=False)
+        fh = f.hits.cpu().numpy().view(np.uint32)
+        bad = np.any(fh != o["hits"], -1)
+        assert not bad.any(), f"frame {k}: {bad.sum()} hit records differ"
+        assert np.array_equal(f.steps.cpu().numpy().view(np.uint32), o["steps"])
+        assert np.max(np.abs(f.shaded.cpu().numpy() - o["shaded"])) <= 1e-5
+        assert (fh[..., 0] != 0xFFFFFFFF).mean() > 0.2
+    w.close()
 
 
 def test_procgen_argument_errors(ctx, atlas):
