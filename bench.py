@@ -583,6 +583,89 @@ def bench_procgen(args):
     ctx.close()
 
 
+def bench_engine(args):
+    """The engine's own frame: Game.init's world (procgen(512, &voxels, 0, 0)
+    into the chunk/pool layout, game.zig:96-97) with the model atlas of its
+    .vox assets (the placed texels kept in tests/golden/atlas_assets.npz),
+    its start camera (player (256, 22, 256) + 3 up, identity rotation, fov
+    pi/2), the shaders' own step caps (192 primary, primary.comp.glsl:43;
+    48 shadow, secondary.comp.glsl:41), 1920x1080.  A step is one vt_render
+    with the world resident in HBM."""
+    import math
+
+    import numpy as np
+    import torch
+
+    import vt
+    from vt import _native as N
+
+    g = np.load(os.path.join(ROOT, "tests", "golden", "atlas_assets.npz"))
+    atlas = np.zeros(256 ** 3, np.uint32)
+    atlas[g["index"]] = g["value"]
+    ctx = vt.Context(0)
+    world = vt.World.empty_brickmap(ctx, 512, atlas)
+    world.procgen()
+    cam = N.Camera()
+    cam.position[:] = [256.0, 25.0, 256.0, 0.0]
+    cam.view[:] = [1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0]
+    cam.fov = float(np.float32(math.pi / 2.0))
+    W, H, sun = 1920, 1080, (0.4, 0.8, 0.3)
+    prm = vt.params(W, H, 192, 48, sun)
+    fs = vt.Frame.alloc(W, H, stats=True)
+    vt.render(ctx, world, cam, fs, prm)
+    torch.cuda.synchronize()
+    st = fs.stats_dict()
+    f = vt.Frame.alloc(W, H)
+    stream = torch.cuda.current_stream()
+    ms = timed(torch, stream, lambda: vt.render(ctx, world, cam, f, prm), args.steps, args.warmup)
+    rays = W * H + st["primary_hits"]
+    cpu = None
+    same = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_ffi as O
+        chunks, pool = world.read_brickmap()
+        ow = O.BrickmapWorld(512, chunks, pool, atlas)
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        t0 = time.perf_counter()
+        o = O.render(ow, cam, W, H, 192, 48, sun, threads=threads, gbuffer=False)
+        dt = time.perf_counter() - t0
+        same = bool(np.array_equal(f.hits.cpu().numpy().view(np.uint32), o["hits"]))
+        cpu = {"value": round((W * H + int(o["stats"][2])) / dt / 1e6, 3), "unit": "Mrays/s",
+               "cores": threads, "kind": "port",
+               "sample": f"the same full frame, {threads} threads, {dt:.2f} s wall"}
+    abytes = 4 * (st["l1_fetches"] + st["l0_fetches"]) + OUT_BYTES_PER_PIXEL * W * H
+    achieved = abytes / (ms * 1e-3) / 1e9
+    line = {
+        "metric": "Mrays/s primary+shadow, the engine's own 1080p frame (Game.init world and "
+                  "camera, shader step caps 192/48)",
+        "value": round(rays / (ms * 1e-3) / 1e6, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "the reference's generator and .vox models",
+        "config": {
+            "workload": "procgen(512, 0, 0) world in the chunk/pool layout + 29-slot model "
+                        "atlas; start camera; 1920x1080; caps 192/48",
+            "primary_hits": st["primary_hits"],
+            "dda_steps": st["primary_steps"] + st["shadow_steps"],
+            "hits_equal_oracle": same,
+        },
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    world.close()
+    ctx.close()
+
+
 def bench_copy(args):
     """Calibration for the HBM counters: one plain device-to-device copy of
     1 GiB (torch's copy kernel) per step -- 1 GiB read + 1 GiB written."""
@@ -608,13 +691,14 @@ def main():
     ap.add_argument("--traversal", choices=["flat", "bitmask"], default="flat")
     ap.add_argument("--workload",
                     choices=["frame", "batched", "diff", "diff_batched", "brickmap", "procgen",
-                             "copy"],
+                             "engine", "copy"],
                     default="frame",
                     help="frame: 512^3, one 1080p view (the headline); batched: 1024^3, 64 "
                          "uneven cameras in one persistent graph launch; diff: 256^3, 16 "
                          "views fwd+bwd; diff_batched: 1024^3 compact albedo, 48 views fwd+bwd; "
                          "brickmap: the headline world in the engine's own layout; "
                          "procgen: the engine's 512^3 world generator; "
+                         "engine: the engine's own frame (its world, camera and step caps); "
                          "copy: 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
@@ -635,6 +719,8 @@ def main():
         return bench_brickmap(args)
     if args.workload == "procgen":
         return bench_procgen(args)
+    if args.workload == "engine":
+        return bench_engine(args)
 
     import numpy as np
     import torch

@@ -952,7 +952,11 @@ __device__ bool trace_general(const BrickmapWorld& w, float ox, float oy, float 
 // the same stopping rule) until a step lands on a nonempty block, leaves the
 // map or reaches the cap; the literal loop then resumes at that block's
 // lookup with the same grid / within / minIdx / step count.  Same hits, steps
-// and positions as trace_general.
+// and positions as trace_general.  The literal loop is an inner loop that a
+// lane leaves when it reaches an empty block: a wave's lanes that leave a
+// model block (or start inside one, as shadow rays do) after different
+// numbers of sub-voxel steps then run their empty-space steps together
+// instead of one group after another.
 // kPositive: the caller guarantees a direction with no negative component (the
 // zero substitution below makes it positive), and the empty-space steps take
 // dense_skip_positive.
@@ -982,101 +986,112 @@ __device__ bool trace_brickmap_fast(const BrickmapWorld& w, float ox, float oy, 
   int s = 0;
   h.data = 0;
   h.hx = h.hy = h.hz = -1.0f;
-  int step;
-  for (step = 0; step < max_steps; step++) {
-    if (g[0] >= B || g[1] >= B || g[2] >= B || g[0] < 0 || g[1] < 0 || g[2] < 0) break;
-    int p[3] = {g[0] + (int)wc[0], g[1] + (int)wc[1], g[2] + (int)wc[2]};
-    int b[3] = {p[0] >> 3, p[1] >> 3, p[2] >> 3};
-    uint32_t block = bm_block(w, b[0], b[1], b[2]);
-    if (block != 0u) {
-      uint32_t sub = bm_sub(w, block, p[0] % 8, p[1] % 8, p[2] % 8);
-      if (sub != 0u) {
-        h.data = sub;
-        h.face = m == 0 ? 2 - pv[0] : (m == 1 ? 4 - pv[1] : 6 - pv[2]);
-        h.bx = b[0];
-        h.by = b[1];
-        h.bz = b[2];
-        h.sub = (uint32_t)(p[0] & 7) | ((uint32_t)(p[1] & 7) << 3) | ((uint32_t)(p[2] & 7) << 6);
-        h.hx = (float)g[0] + wc[0];
-        h.hy = (float)g[1] + wc[1];
-        h.hz = (float)g[2] + wc[2];
-        h.steps = step + 1;
-        return true;
-      }
-      if (s != 0) {
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-          g[i] += (int)wc[i];
-          wc[i] = wc[i] - floorf(wc[i]);
+  int step = 0;
+  bool hit = false;
+  for (;;) {
+    // The literal loop (both levels) until the lane stands at an empty
+    // block at the block level, hits, leaves the map or reaches the cap.
+    // Lanes at an empty block wait here for the rest of the wave, so the
+    // wave's empty-space runs below start together.
+    bool fast = false;
+    for (;;) {
+      if (step >= max_steps || g[0] >= B || g[1] >= B || g[2] >= B || g[0] < 0 || g[1] < 0 ||
+          g[2] < 0)
+        break;
+      int p[3] = {g[0] + (int)wc[0], g[1] + (int)wc[1], g[2] + (int)wc[2]};
+      int b[3] = {p[0] >> 3, p[1] >> 3, p[2] >> 3};
+      uint32_t block = bm_block(w, b[0], b[1], b[2]);
+      if (block != 0u) {
+        uint32_t sub = bm_sub(w, block, p[0] % 8, p[1] % 8, p[2] % 8);
+        if (sub != 0u) {
+          h.data = sub;
+          h.face = m == 0 ? 2 - pv[0] : (m == 1 ? 4 - pv[1] : 6 - pv[2]);
+          h.bx = b[0];
+          h.by = b[1];
+          h.bz = b[2];
+          h.sub = (uint32_t)(p[0] & 7) | ((uint32_t)(p[1] & 7) << 3) | ((uint32_t)(p[2] & 7) << 6);
+          h.hx = (float)g[0] + wc[0];
+          h.hy = (float)g[1] + wc[1];
+          h.hz = (float)g[2] + wc[2];
+          h.steps = step + 1;
+          hit = true;
+          break;
         }
-        s = 0;
-      }
-    } else {
-      if (s != 3) {
+        if (s != 0) {
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-          int lo = g[i] & 7;
-          wc[i] = wc[i] + (float)lo;
-          g[i] -= lo;
+          for (int i = 0; i < 3; i++) {
+            g[i] += (int)wc[i];
+            wc[i] = wc[i] - floorf(wc[i]);
+          }
+          s = 0;
         }
-        s = 3;
-      }
-      // empty block at the block level: the fast loop takes the next steps
-      DenseCell s0;
-      s0.wx = wc[0];
-      s0.wy = wc[1];
-      s0.wz = wc[2];
-      s0.c = morton3((uint32_t)g[0] >> 3, (uint32_t)g[1] >> 3, (uint32_t)g[2] >> 3) ^ flip;
-      const uint32_t mx = in_vgpr(w.mx), my = in_vgpr(w.my), mz = in_vgpr(w.mz);
-      int s0steps = step;
-      if constexpr (kPositive)
-        dense_skip_positive<VT_SKIP_STEPS>(d[0], d[1], d[2], inv[0], inv[1], inv[2], mx, my, mz,
-                                           s0, s0steps, max_steps, occ);
-      else
-        dense_skip_empty<VT_SKIP_STEPS>(P3[0], P3[1], P3[2], A3[0], A3[1], A3[2], d[0], d[1],
-                                        d[2], inv[0], inv[1], inv[2], mx, my, mz, flip, s0,
-                                        s0steps, max_steps, occ);
-      float fw[3];
-      uint32_t fc, fM, fa, fv;
-      int sF;
-      bool wrappedF;
-      dense_loop_prefetch(P3[0], P3[1], P3[2], A3[0], A3[1], A3[2], d[0], d[1], d[2], inv[0],
-                          inv[1], inv[2], mx, my, mz, flip, s0, s0steps, max_steps, occ, fw[0],
-                          fw[1], fw[2], fc, fM, fa, fv, sF, wrappedF);
-      (void)fa;
-      (void)fv;
-      if (wrappedF) {  // left the map: the bounds test of iteration sF
-        h.steps = sF;
-        return false;
-      }
-      const uint32_t gc = fc ^ flip;
-      g[0] = (int)(compact1by2(gc) << 3);
-      g[1] = (int)(compact1by2(gc >> 1) << 3);
-      g[2] = (int)(compact1by2(gc >> 2) << 3);
-      wc[0] = fw[0];
-      wc[1] = fw[1];
-      wc[2] = fw[2];
-      m = fM == mx ? 0 : (fM == my ? 1 : 2);
-      step = sF - 1;  // iteration sF starts with its bounds test and lookup
-      continue;
-    }
-    float t[3];
+      } else {
+        if (s != 3) {
 #pragma unroll
-    for (int i = 0; i < 3; i++) t[i] = ((float)(pv[i] << s) - wc[i]) * inv[i];
-    m = t[0] < t[1] ? (t[0] < t[2] ? 0 : 2) : (t[1] < t[2] ? 1 : 2);
-    float tm = m == 0 ? t[0] : (m == 1 ? t[1] : t[2]);
-    int sm = m == 0 ? sgn[0] : (m == 1 ? sgn[1] : sgn[2]);
-    int pm = m == 0 ? pv[0] : (m == 1 ? pv[1] : pv[2]);
-    float am = (float)((1 - pm) << s) * 0.999f;
+          for (int i = 0; i < 3; i++) {
+            int lo = g[i] & 7;
+            wc[i] = wc[i] + (float)lo;
+            g[i] -= lo;
+          }
+          s = 3;
+        }
+        fast = true;  // this iteration's step onwards run in the loop below
+        break;
+      }
+      float t[3];
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-      float nw = wc[i] + d[i] * tm;
-      wc[i] = (i == m) ? am : nw;
-      g[i] += (i == m) ? sm * (1 << s) : 0;
+      for (int i = 0; i < 3; i++) t[i] = ((float)(pv[i] << s) - wc[i]) * inv[i];
+      m = t[0] < t[1] ? (t[0] < t[2] ? 0 : 2) : (t[1] < t[2] ? 1 : 2);
+      float tm = m == 0 ? t[0] : (m == 1 ? t[1] : t[2]);
+      int sm = m == 0 ? sgn[0] : (m == 1 ? sgn[1] : sgn[2]);
+      int pm = m == 0 ? pv[0] : (m == 1 ? pv[1] : pv[2]);
+      float am = (float)((1 - pm) << s) * 0.999f;
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        float nw = wc[i] + d[i] * tm;
+        wc[i] = (i == m) ? am : nw;
+        g[i] += (i == m) ? sm * (1 << s) : 0;
+      }
+      step++;
     }
+    if (!fast) break;
+    // empty block at the block level: the fast loop takes the next steps
+    DenseCell s0;
+    s0.wx = wc[0];
+    s0.wy = wc[1];
+    s0.wz = wc[2];
+    s0.c = morton3((uint32_t)g[0] >> 3, (uint32_t)g[1] >> 3, (uint32_t)g[2] >> 3) ^ flip;
+    const uint32_t mx = in_vgpr(w.mx), my = in_vgpr(w.my), mz = in_vgpr(w.mz);
+    int s0steps = step;
+    if constexpr (kPositive)
+      dense_skip_positive<VT_SKIP_STEPS>(d[0], d[1], d[2], inv[0], inv[1], inv[2], mx, my, mz, s0,
+                                         s0steps, max_steps, occ);
+    else
+      dense_skip_empty<VT_SKIP_STEPS>(P3[0], P3[1], P3[2], A3[0], A3[1], A3[2], d[0], d[1], d[2],
+                                      inv[0], inv[1], inv[2], mx, my, mz, flip, s0, s0steps,
+                                      max_steps, occ);
+    float fw[3];
+    uint32_t fc, fM, fa, fv;
+    int sF;
+    bool wrappedF;
+    dense_loop_prefetch(P3[0], P3[1], P3[2], A3[0], A3[1], A3[2], d[0], d[1], d[2], inv[0],
+                        inv[1], inv[2], mx, my, mz, flip, s0, s0steps, max_steps, occ, fw[0],
+                        fw[1], fw[2], fc, fM, fa, fv, sF, wrappedF);
+    (void)fa;
+    (void)fv;
+    step = sF;  // iteration sF starts with its bounds test and lookup
+    if (wrappedF) break;  // left the map: the bounds test of iteration sF
+    const uint32_t gc = fc ^ flip;
+    g[0] = (int)(compact1by2(gc) << 3);
+    g[1] = (int)(compact1by2(gc >> 1) << 3);
+    g[2] = (int)(compact1by2(gc >> 2) << 3);
+    wc[0] = fw[0];
+    wc[1] = fw[1];
+    wc[2] = fw[2];
+    m = fM == mx ? 0 : (fM == my ? 1 : 2);
   }
-  h.steps = step;
-  return false;
+  if (!hit) h.steps = step;
+  return hit;
 }
 
 }  // namespace vt
