@@ -3,8 +3,9 @@
 * The chunk table and block pool equal, word for word, those the oracle's
   literal restatement of procgen.zig leaves in a VoxelBrickmap (allocation
   order included): the engine's 512^3 map at the offsets Game.init uses
-  (0, 0), at offsets that reach negative noise coordinates, at 1024^3, and
-  at small maps (no trees).  A map on which a tree would leave the bounds
+  (0, 0), at offsets that reach negative noise coordinates, at 1024^3, at
+  504^3 (a map that is not a power of two: no occupancy bytes), and at
+  small maps (no trees).  A map on which a tree would leave the bounds
   (320^3) is rejected and left unchanged, as the oracle flags it.
 * The generated world (with the engine's 29-slot model atlas) renders
   exactly like a world created from the oracle's buffers, and like the
@@ -35,7 +36,8 @@ def atlas():
 
 
 @pytest.mark.parametrize("dim,ox,oy", [(512, 0.0, 0.0), (512, -37.25, 1000.5), (1024, 0.0, 0.0),
-                                       (256, 40.0, -20.0), (64, 3.0, 7.0), (16, 0.0, 0.0)])
+                                       (504, 0.0, 0.0), (256, 40.0, -20.0), (64, 3.0, 7.0),
+                                       (16, 0.0, 0.0)])
 def test_procgen_equals_oracle(ctx, atlas, dim, ox, oy):
     rc, chunks, pool = O.procgen(dim, ox, oy)
     assert rc == 0
