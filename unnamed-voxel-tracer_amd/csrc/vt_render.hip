@@ -16,10 +16,15 @@
 //
 // Shadow rays come in two kinds: most of those not lit are occluded after a
 // step or two, most lit ones run to the edge of the grid (hundreds of steps).
-// A second kernel over the long ones alone (a job list, 64 long rays to a
-// wave) was built and measured: with the XCD strips it was slower than
-// this single kernel at every frame size from 256^2 to 2560x1440, so the
-// shadow ray stays in the pixel's lane.
+// In a wave that holds both, the occluded lanes idle while the lit ones run.
+// So (reference composite, flat lookup) each lane runs its shadow ray for a
+// first segment of one empty-space iteration; the rays still running are
+// then parked in LDS, packed, and dealt out to the block's waves from the
+// first on, 64 to a wave, which finish them -- the block's waves without a
+// parked ray end there.  Same steps and results per ray; one block-wide
+// barrier on each side of the packing.  (A second kernel over the long rays
+// of the whole frame, through a job list in memory, was built and measured
+// slower than the single kernel once the XCD strips were in.)
 #include "vt_pixel.hpp"
 
 namespace vt {
@@ -37,6 +42,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   const uint32_t px = bx * (2u * tw) + (wave & 1u) * tw + (lane & (tw - 1u));
   const uint32_t py = by * (2u * th) + (wave >> 1) * th + (lane >> log2_tw);
   Counters cnt = {0, 0, 0, 0, 0, 0, 0, 0};
+  if constexpr (kShade == kShadeRef && L::kPrefetch && !kStats) {
+    // every thread of the block reaches both barriers: no early exit above
+    __shared__ ShadowJob jobs[256];
+    __shared__ uint32_t wave_jobs[4];
+    ShadowJob job;
+    bool surv = false;
+    if (px < v.W && py < v.H) surv = trace_pixel_dense_split<L>(w, v, sc, out, px, py, cnt, job);
+    const unsigned long long bal = __ballot(surv);
+    if (lane == 0) wave_jobs[wave] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t base = 0, total = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++) {
+      const uint32_t n = wave_jobs[i];
+      base += i < wave ? n : 0u;
+      total += n;
+    }
+    if (surv) jobs[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = job;
+    __syncthreads();
+    if (threadIdx.x < total) finish_shadow_job<L>(w, v, sc, out, jobs[threadIdx.x], cnt);
+    return;
+  }
   GradAcc gacc = grad_acc();
   if (px < v.W && py < v.H)
     trace_pixel_dense<L, kStats, kShade>(w, v, sc, out, px, py, cnt, gacc);

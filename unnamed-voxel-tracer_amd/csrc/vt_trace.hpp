@@ -743,6 +743,74 @@ __device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const Den
   h.steps = sF + 1;
 }
 
+// dense_run_prefetch with the empty-space phase limited to seg_limit: returns
+// false (r then holds the state to continue from: within, code, steps --
+// an iteration start whose lookups were all empty) when that phase ran out
+// of its limit before the ray stopped and before the ray's cap; true with h
+// filled otherwise.  A second call from that state with a limit at or above
+// the cap takes the same iterations as one uncut call.
+template <class L, bool kPositive = false>
+__device__ __forceinline__ bool dense_run_prefetch_seg(DenseRay& r, L& lk, const DenseWorld& w,
+                                                       DenseHit& h, int seg_limit) {
+  const uint32_t mx = in_vgpr(w.mx), my = in_vgpr(w.my), mz = in_vgpr(w.mz), flip = r.flip;
+  const int max_steps = r.max_steps;
+  const int limit = seg_limit < max_steps ? seg_limit : max_steps;
+  DenseCell s0;
+  s0.wx = r.wx;
+  s0.wy = r.wy;
+  s0.wz = r.wz;
+  s0.c = r.c;
+  const __amdgpu_buffer_rsrc_t vox =
+      __builtin_amdgcn_make_buffer_rsrc((void*)w.vox, (short)0, (int)w.nvox, 0x00020000);
+  int steps0 = r.steps;
+  bool blocked;
+  if constexpr (kPositive)
+    blocked = dense_skip_positive<VT_SKIP_STEPS>(r.dx, r.dy, r.dz, r.ix, r.iy, r.iz, mx, my, mz,
+                                                 s0, steps0, limit, vox);
+  else
+    blocked = dense_skip_empty<VT_SKIP_STEPS>(r.Px, r.Py, r.Pz, r.Ax, r.Ay, r.Az, r.dx, r.dy,
+                                              r.dz, r.ix, r.iy, r.iz, mx, my, mz, flip, s0,
+                                              steps0, limit, vox);
+  if (!blocked && limit < max_steps) {
+    r.wx = s0.wx;
+    r.wy = s0.wy;
+    r.wz = s0.wz;
+    r.c = s0.c;
+    r.steps = steps0;
+    return false;
+  }
+  float fwx, fwy, fwz;
+  uint32_t fc, fM, fa, fv;
+  int sF;
+  bool wrappedF;
+  dense_loop_prefetch(r.Px, r.Py, r.Pz, r.Ax, r.Ay, r.Az, r.dx, r.dy, r.dz, r.ix, r.iy, r.iz, mx,
+                      my, mz, flip, s0, steps0, max_steps, vox, fwx, fwy, fwz, fc, fM, fa, fv,
+                      sF, wrappedF);
+  (void)lk;
+  if (sF >= max_steps || wrappedF) {
+    h.hit = false;
+    h.steps = sF;
+    return true;
+  }
+  const uint32_t g = fc ^ flip;
+  const bool fx = fM == mx, fy = fM == my;
+  const int pm = (flip & fM) ? 0 : 1;
+  h.hit = true;
+  h.face = (fx ? 2 : (fy ? 4 : 6)) - pm;
+  h.bx = (int)compact1by2(fa);
+  h.by = (int)compact1by2(fa >> 1);
+  h.bz = (int)compact1by2(fa >> 2);
+  h.code = fa;
+  h.v = fv;
+  h.sub = (uint32_t)((int)fwx & 7) | ((uint32_t)((int)fwy & 7) << 3) |
+          ((uint32_t)((int)fwz & 7) << 6);
+  h.hx = (float)(compact1by2(g) << 3) + fwx;
+  h.hy = (float)(compact1by2(g >> 1) << 3) + fwy;
+  h.hz = (float)(compact1by2(g >> 2) << 3) + fwz;
+  h.steps = sF + 1;
+  return true;
+}
+
 template <bool kStats, class L>
 __device__ __forceinline__ void dense_run(DenseRay& r, L& lk, const DenseWorld& w, DenseHit& h,
                                           uint32_t& l1f, uint32_t& l0f) {
