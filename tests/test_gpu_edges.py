@@ -6,7 +6,8 @@ away (all sky), exactly on lattice points, inside a solid voxel, looking
 exactly along each axis (zero direction components: the 0.001
 substitution), grazing a face; images 1x1, 1xN, Nx1 and ragged tile edges;
 step caps 0, 1, 2 and a zero shadow cap; both traversals and the reference
-layout of the same worlds.  The bar is the one of the main parity tests:
+layout of the same worlds; frames large enough for the packed long shadow
+rays.  The bar is the one of the main parity tests:
 hit records, step counts, G-buffer position/normal/illumination and the
 statistics counters bit-exact, colours within 1e-5.
 """
@@ -164,3 +165,30 @@ def test_edge_diagonal_cameras(ctx):
             o = O.render(ow, cam, W, H, 512, 512, (0.5, 0.5, 0.5))
             compare(g, o, label=f"{cname} {W}x{H}")
     w.close()
+
+
+def test_edge_packed_shadow_rays(ctx):
+    """Frames with enough blocks (16x16 pixels each) to fill every CU's 8
+    block slots, for which the kernel packs each block's long shadow rays
+    after a first segment of one empty-space iteration and finishes them 64
+    to a wave: tied suns (two- and three-axis ties every step), suns of either
+    sign, shadow caps around the segment's end (5, 6, 7, 11 steps) and
+    lattice-point cameras, against the oracle."""
+    W, H = 1280, 720
+    cus = ctx.info()["compute_units"]
+    assert ((W + 15) // 16) * ((H + 15) // 16) >= 8 * cus
+    pal = S.palette()
+    cases = [("checker32", (0.5, 0.5, 0.5), 256), ("checker32", (-0.5, 0.5, 0.5), 256),
+             ("sparse64", (1.0, 1.0, 0.0), 512), ("sparse64", S.SUN, 5),
+             ("sparse64", S.SUN, 6), ("sparse64", (0.25, 0.5, 0.25), 7),
+             ("sparse64", (-0.5, 0.7, -0.2), 11), ("corners32", S.SUN, 256)]
+    for wname, sun, cap in cases:
+        grid = WORLDS[wname]
+        n = grid.shape[0]
+        w = vt.World.dense(ctx, grid, pal)
+        ow = O.DenseWorld(grid, pal)
+        for cname, cam in (cameras(n)[1], cameras(n)[7]):
+            g = gpu_render(ctx, w, cam, W, H, 8 * n, cap, sun=sun)
+            o = O.render(ow, cam, W, H, 8 * n, cap, sun)
+            compare(g, o, label=f"packed {wname} {cname} sun={sun} shadow cap {cap}")
+        w.close()

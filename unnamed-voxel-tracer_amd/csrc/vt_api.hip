@@ -429,7 +429,7 @@ int vt_render(vt_ctx* ctx, const vt_world* world, const vt_camera* camera,
   hipError_t e;
   if (world->kind == WorldKind::Dense)
     e = vt::launch_render_dense(vt::dense_view(world), p.traversal, v, sc, o, tile,
-                                (hipStream_t)stream);
+                                (uint32_t)ctx->compute_units, (hipStream_t)stream);
   else
     e = vt::launch_render_brickmap(vt::brickmap_view(world), v, sc, o, tile, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "vt_render: launch");

@@ -100,7 +100,7 @@ FrameOut frame_out(const vt_frame* frame);
 
 hipError_t launch_render_dense(const DenseWorld& w, uint32_t traversal, const ViewConsts& v,
                                const ShadeConsts& sc, const FrameOut& out, TileShape tile,
-                               hipStream_t stream);
+                               uint32_t compute_units, hipStream_t stream);
 hipError_t launch_render_brickmap(const BrickmapWorld& w, const ViewConsts& v,
                                   const ShadeConsts& sc, const FrameOut& out, TileShape tile,
                                   hipStream_t stream);
