@@ -169,11 +169,12 @@ def test_edge_diagonal_cameras(ctx):
 
 def test_edge_packed_shadow_rays(ctx):
     """Frames with enough blocks (16x16 pixels each) to fill every CU's 8
-    block slots, for which the kernel packs each block's long shadow rays
-    after a first segment of one empty-space iteration and finishes them 64
-    to a wave: tied suns (two- and three-axis ties every step), suns of either
-    sign, shadow caps around the segment's end (5, 6, 7, 11 steps) and
-    lattice-point cameras, against the oracle."""
+    block slots, for which the kernels pack each block's long shadow rays
+    after a first segment of one empty-space iteration and finish them 64
+    to a wave (shadow caps of at least 256 steps; the smaller caps here take
+    the unpacked path and the segment's end): tied suns (two- and three-axis
+    ties every step), suns of either sign, lattice-point cameras, on the
+    dense and the reference layout, against the oracle."""
     W, H = 1280, 720
     cus = ctx.info()["compute_units"]
     assert ((W + 15) // 16) * ((H + 15) // 16) >= 8 * cus
@@ -187,8 +188,16 @@ def test_edge_packed_shadow_rays(ctx):
         n = grid.shape[0]
         w = vt.World.dense(ctx, grid, pal)
         ow = O.DenseWorld(grid, pal)
+        dim, chunks, pool, atlas = BM.from_dense(grid, pal)
+        wb = vt.World.brickmap(ctx, dim, chunks, pool, atlas)
+        owb = O.BrickmapWorld(dim, chunks, pool, atlas)
         for cname, cam in (cameras(n)[1], cameras(n)[7]):
             g = gpu_render(ctx, w, cam, W, H, 8 * n, cap, sun=sun)
             o = O.render(ow, cam, W, H, 8 * n, cap, sun)
             compare(g, o, label=f"packed {wname} {cname} sun={sun} shadow cap {cap}")
+            # the reference layout: sub-voxel steps inside the hit block first
+            g = gpu_render(ctx, wb, cam, W, H, 64 * n, 8 * cap, sun=sun)
+            o = O.render(owb, cam, W, H, 64 * n, 8 * cap, sun)
+            compare(g, o, stats=False, label=f"packed brickmap {wname} {cname} sun={sun}")
         w.close()
+        wb.close()

@@ -431,7 +431,8 @@ int vt_render(vt_ctx* ctx, const vt_world* world, const vt_camera* camera,
     e = vt::launch_render_dense(vt::dense_view(world), p.traversal, v, sc, o, tile,
                                 (uint32_t)ctx->compute_units, (hipStream_t)stream);
   else
-    e = vt::launch_render_brickmap(vt::brickmap_view(world), v, sc, o, tile, (hipStream_t)stream);
+    e = vt::launch_render_brickmap(vt::brickmap_view(world), v, sc, o, tile,
+                                   (uint32_t)ctx->compute_units, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "vt_render: launch");
   return VT_OK;
 }

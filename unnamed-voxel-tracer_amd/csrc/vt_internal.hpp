@@ -103,7 +103,7 @@ hipError_t launch_render_dense(const DenseWorld& w, uint32_t traversal, const Vi
                                uint32_t compute_units, hipStream_t stream);
 hipError_t launch_render_brickmap(const BrickmapWorld& w, const ViewConsts& v,
                                   const ShadeConsts& sc, const FrameOut& out, TileShape tile,
-                                  hipStream_t stream);
+                                  uint32_t compute_units, hipStream_t stream);
 
 // World construction kernels.
 // occ[morton(b)] = (map_getVoxel(b) != 0) for every block b of a brickmap

@@ -27,10 +27,18 @@
 // slower than the single kernel once the XCD strips were in.)
 #include "vt_pixel.hpp"
 
-// Frames with at least this many 256-thread blocks per CU take the packed
-// shadow rays (k_render_dense's kPack).
+// Frames with at least this many 256-thread blocks per CU, and a shadow cap
+// of at least VT_PACK_MIN_SHADOW_CAP steps, take the packed shadow rays
+// (kPack of k_render_dense and k_render_brickmap).  The packing costs two
+// block-wide barriers: it pays when the frame fills every CU's block slots
+// (1080p faster, 512x512 views slower) and when shadow rays can run long --
+// the imbalance it removes is bounded by the shadow cap (the engine's own
+// frame, cap 48, measured 16% slower with it in the brickmap kernel).
 #ifndef VT_PACK_BLOCKS_PER_CU
 #define VT_PACK_BLOCKS_PER_CU 8u
+#endif
+#ifndef VT_PACK_MIN_SHADOW_CAP
+#define VT_PACK_MIN_SHADOW_CAP 256
 #endif
 
 namespace vt {
@@ -83,7 +91,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
 }
 
 // Same 8-wave budget as k_render_dense (0.97 vs 1.02 ms at 90 VGPRs / 5 waves).
-template <bool kStats>
+// kPack: the block packs its long shadow rays after a first segment and
+// finishes them 64 to a wave, as k_render_dense does (worlds with occupancy
+// bytes, no statistics).
+template <bool kStats, bool kPack = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_render_brickmap(
     BrickmapWorld w, ViewConsts v, ShadeConsts sc, FrameOut out, uint32_t log2_tw,
     uint32_t blocks_x) {
@@ -95,6 +106,110 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   const uint32_t py = by * (2u * th) + (wave >> 1) * th + (lane >> log2_tw);
   const bool active = px < v.W && py < v.H;
   Counters cnt = {0, 0, 0, 0, 0, 0, 0, 0};
+  if constexpr (kPack && !kStats) {
+    // (launched only for worlds with occupancy bytes) every thread of the
+    // block reaches both barriers
+    __shared__ ShadowJob jobs[256];
+    __shared__ uint32_t wave_jobs[4];
+    ShadowJob job;
+    bool surv = false;
+    const bool pos = sc.sun[0] >= 0.0f && sc.sun[1] >= 0.0f && sc.sun[2] >= 0.0f;
+    if (active) {
+      float dx, dy, dz;
+      raygen(v, px, py, dx, dy, dz);
+      float tn, tf;
+      const float D = (float)w.dim;
+      intersect_aabb(v.pos[0], v.pos[1], v.pos[2], dx, dy, dz, 0.0f, 0.0f, 0.0f, D, D, D, tn, tf);
+      float tc = fmaxf(tn, 0.0f);
+      float ex = (v.pos[0] + dx * tc) - 0.001f;
+      float ey = (v.pos[1] + dy * tc) - 0.001f;
+      float ez = (v.pos[2] + dz * tc) - 0.001f;
+      GeneralHit g;
+      trace_brickmap_fast(w, ex, ey, ez, dx, dy, dz, sc.max_primary, g);
+      Primary p;
+      p.hit = g.data != 0u;
+      p.steps = g.steps;
+      if (p.hit) {
+        p.face = g.face;
+        p.voxel = (uint32_t)g.bx + w.dim * ((uint32_t)g.by + w.dim * (uint32_t)g.bz);
+        p.data = g.data;
+        p.material = bm_block(w, g.bx, g.by, g.bz) & 255u;
+        p.sub = g.sub;
+        p.hx = g.hx;
+        p.hy = g.hy;
+        p.hz = g.hz;
+        float sox, soy, soz;
+        shadow_origin(p, sc.flags, sox, soy, soz);
+        store_hit_primary(v, sc, out, (size_t)py * v.W + px, p);
+        const uint32_t data = p.data, info0 = hit_info(p.face, p.sub, p.material, false);
+        GeneralHit sh;
+        BrickPark park;
+        if (pos)
+          trace_brickmap_fast<true, true>(w, sox, soy, soz, sc.sun[0], sc.sun[1], sc.sun[2],
+                                          sc.max_shadow, sh, &park);
+        else
+          trace_brickmap_fast<false, true>(w, sox, soy, soz, sc.sun[0], sc.sun[1], sc.sun[2],
+                                           sc.max_shadow, sh, &park);
+        if (sh.steps < 0) {
+          surv = true;
+          job.wx = park.wx;
+          job.wy = park.wy;
+          job.wz = park.wz;
+          job.c = park.c;
+          job.steps = park.steps;
+          job.pxy = px | (py << 16);
+          job.data = data;
+          job.info0 = info0;
+          job.psteps = p.steps;
+          job.sox = sox;
+          job.soy = soy;
+          job.soz = soz;
+        } else {
+          const bool shit = sh.data != 0u;
+          store_hit_shadow_ref(v, sc, out, px, py, data, info0, p.steps, shit,
+                               shit ? sh.hx : -1.0f, shit ? sh.hy : -1.0f, shit ? sh.hz : -1.0f,
+                               sox, soy, soz, sh.steps, cnt);
+        }
+      } else {
+        GradAcc gacc = grad_acc();
+        write_pixel<kShadeRef>(v, sc, out, px, py, p, false, -1.0f, -1.0f, -1.0f, 0.f, 0.f, 0.f, 0,
+                               cnt, nullptr, gacc);
+      }
+    }
+    const unsigned long long bal = __ballot(surv);
+    if (lane == 0) wave_jobs[wave] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t base = 0, total = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++) {
+      const uint32_t n = wave_jobs[i];
+      base += i < wave ? n : 0u;
+      total += n;
+    }
+    if (surv) jobs[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = job;
+    __syncthreads();
+    if (threadIdx.x < total) {
+      const ShadowJob& jb = jobs[threadIdx.x];
+      BrickPark from;
+      from.wx = jb.wx;
+      from.wy = jb.wy;
+      from.wz = jb.wz;
+      from.c = jb.c;
+      from.steps = jb.steps;
+      GeneralHit sh;
+      if (pos)
+        trace_brickmap_fast<true, false, true>(w, 0.f, 0.f, 0.f, sc.sun[0], sc.sun[1], sc.sun[2],
+                                               sc.max_shadow, sh, nullptr, &from);
+      else
+        trace_brickmap_fast<false, false, true>(w, 0.f, 0.f, 0.f, sc.sun[0], sc.sun[1],
+                                                sc.sun[2], sc.max_shadow, sh, nullptr, &from);
+      const bool shit = sh.data != 0u;
+      store_hit_shadow_ref(v, sc, out, jb.pxy & 0xFFFFu, jb.pxy >> 16, jb.data, jb.info0,
+                           jb.psteps, shit, shit ? sh.hx : -1.0f, shit ? sh.hy : -1.0f,
+                           shit ? sh.hz : -1.0f, jb.sox, jb.soy, jb.soz, sh.steps, cnt);
+    }
+    return;
+  }
   if (active) {
     float dx, dy, dz;
     raygen(v, px, py, dx, dy, dz);
@@ -200,10 +315,8 @@ hipError_t launch_render_dense(const DenseWorld& w, uint32_t traversal, const Vi
                                uint32_t compute_units, hipStream_t stream) {
   uint32_t blocks_x, nblocks;
   grid_shape(v, tile, blocks_x, nblocks);
-  // Packing the long shadow rays costs two block-wide barriers; it pays when
-  // the frame has blocks enough to fill every CU's 8 block slots
-  // (measured: 1080p faster, 512x512 and smaller views slower).
-  const bool pack = nblocks >= VT_PACK_BLOCKS_PER_CU * compute_units;
+  const bool pack = nblocks >= VT_PACK_BLOCKS_PER_CU * compute_units &&
+                    sc.max_shadow >= VT_PACK_MIN_SHADOW_CAP;
   const dim3 grid(nblocks);
   const bool stats = out.stats != nullptr;
   if (traversal == VT_TRAVERSAL_BITMASK) {
@@ -222,12 +335,16 @@ hipError_t launch_render_dense(const DenseWorld& w, uint32_t traversal, const Vi
 
 hipError_t launch_render_brickmap(const BrickmapWorld& w, const ViewConsts& v,
                                   const ShadeConsts& sc, const FrameOut& out, TileShape tile,
-                                  hipStream_t stream) {
+                                  uint32_t compute_units, hipStream_t stream) {
   uint32_t blocks_x, nblocks;
   grid_shape(v, tile, blocks_x, nblocks);
   dim3 grid(nblocks), block(256);
   if (out.stats)
     hipLaunchKernelGGL((k_render_brickmap<true>), grid, block, 0, stream, w, v, sc, out,
+                       tile.log2_w, blocks_x);
+  else if (w.occ && nblocks >= VT_PACK_BLOCKS_PER_CU * compute_units &&
+           sc.max_shadow >= VT_PACK_MIN_SHADOW_CAP)
+    hipLaunchKernelGGL((k_render_brickmap<false, true>), grid, block, 0, stream, w, v, sc, out,
                        tile.log2_w, blocks_x);
   else
     hipLaunchKernelGGL((k_render_brickmap<false>), grid, block, 0, stream, w, v, sc, out,

@@ -1028,9 +1028,23 @@ __device__ bool trace_general(const BrickmapWorld& w, float ox, float oy, float 
 // kPositive: the caller guarantees a direction with no negative component (the
 // zero substitution below makes it positive), and the empty-space steps take
 // dense_skip_positive.
-template <bool kPositive = false>
+// A ray's state at the start of an empty-space iteration (block level, every
+// lookup so far empty): the point where a shadow ray can be parked and
+// resumed with the same steps.
+struct BrickPark {
+  float wx, wy, wz;
+  uint32_t c;  // flipped Morton code of gridsCoords >> 3
+  int steps;
+};
+
+// kSeg: the first empty-space run stops after one iteration; a ray still
+// running then is parked in *park (h.steps = -1) instead of finished.
+// kResume: continue the ray parked in *from (ox, oy, oz are then unused).
+// Without either, the traversal compiles as it did before they existed.
+template <bool kPositive = false, bool kSeg = false, bool kResume = false>
 __device__ bool trace_brickmap_fast(const BrickmapWorld& w, float ox, float oy, float oz, float dx,
-                                    float dy, float dz, int max_steps, GeneralHit& h) {
+                                    float dy, float dz, int max_steps, GeneralHit& h,
+                                    BrickPark* park = nullptr, const BrickPark* from = nullptr) {
   if (dx == 0.0f) dx = 0.001f;
   if (dy == 0.0f) dy = 0.001f;
   if (dz == 0.0f) dz = 0.001f;
@@ -1056,71 +1070,90 @@ __device__ bool trace_brickmap_fast(const BrickmapWorld& w, float ox, float oy, 
   h.hx = h.hy = h.hz = -1.0f;
   int step = 0;
   bool hit = false;
+  bool resume = false;
+  if constexpr (kResume) {
+    const uint32_t gc0 = from->c ^ flip;
+    g[0] = (int)(compact1by2(gc0) << 3);
+    g[1] = (int)(compact1by2(gc0 >> 1) << 3);
+    g[2] = (int)(compact1by2(gc0 >> 2) << 3);
+    wc[0] = from->wx;
+    wc[1] = from->wy;
+    wc[2] = from->wz;
+    s = 3;
+    step = from->steps;
+    resume = true;
+  }
+  bool first_run = true;
   for (;;) {
     // The literal loop (both levels) until the lane stands at an empty
     // block at the block level, hits, leaves the map or reaches the cap.
     // Lanes at an empty block wait here for the rest of the wave, so the
     // wave's empty-space runs below start together.
     bool fast = false;
-    for (;;) {
-      if (step >= max_steps || g[0] >= B || g[1] >= B || g[2] >= B || g[0] < 0 || g[1] < 0 ||
-          g[2] < 0)
-        break;
-      int p[3] = {g[0] + (int)wc[0], g[1] + (int)wc[1], g[2] + (int)wc[2]};
-      int b[3] = {p[0] >> 3, p[1] >> 3, p[2] >> 3};
-      uint32_t block = bm_block(w, b[0], b[1], b[2]);
-      if (block != 0u) {
-        uint32_t sub = bm_sub(w, block, p[0] % 8, p[1] % 8, p[2] % 8);
-        if (sub != 0u) {
-          h.data = sub;
-          h.face = m == 0 ? 2 - pv[0] : (m == 1 ? 4 - pv[1] : 6 - pv[2]);
-          h.bx = b[0];
-          h.by = b[1];
-          h.bz = b[2];
-          h.sub = (uint32_t)(p[0] & 7) | ((uint32_t)(p[1] & 7) << 3) | ((uint32_t)(p[2] & 7) << 6);
-          h.hx = (float)g[0] + wc[0];
-          h.hy = (float)g[1] + wc[1];
-          h.hz = (float)g[2] + wc[2];
-          h.steps = step + 1;
-          hit = true;
+    if (kResume && resume) {  // straight to the empty-space run
+      fast = true;
+      resume = false;
+    } else {
+      for (;;) {
+        if (step >= max_steps || g[0] >= B || g[1] >= B || g[2] >= B || g[0] < 0 || g[1] < 0 ||
+            g[2] < 0)
+          break;
+        int p[3] = {g[0] + (int)wc[0], g[1] + (int)wc[1], g[2] + (int)wc[2]};
+        int b[3] = {p[0] >> 3, p[1] >> 3, p[2] >> 3};
+        uint32_t block = bm_block(w, b[0], b[1], b[2]);
+        if (block != 0u) {
+          uint32_t sub = bm_sub(w, block, p[0] % 8, p[1] % 8, p[2] % 8);
+          if (sub != 0u) {
+            h.data = sub;
+            h.face = m == 0 ? 2 - pv[0] : (m == 1 ? 4 - pv[1] : 6 - pv[2]);
+            h.bx = b[0];
+            h.by = b[1];
+            h.bz = b[2];
+            h.sub = (uint32_t)(p[0] & 7) | ((uint32_t)(p[1] & 7) << 3) | ((uint32_t)(p[2] & 7) << 6);
+            h.hx = (float)g[0] + wc[0];
+            h.hy = (float)g[1] + wc[1];
+            h.hz = (float)g[2] + wc[2];
+            h.steps = step + 1;
+            hit = true;
+            break;
+          }
+          if (s != 0) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+              g[i] += (int)wc[i];
+              wc[i] = wc[i] - floorf(wc[i]);
+            }
+            s = 0;
+          }
+        } else {
+          if (s != 3) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+              int lo = g[i] & 7;
+              wc[i] = wc[i] + (float)lo;
+              g[i] -= lo;
+            }
+            s = 3;
+          }
+          fast = true;  // this iteration's step onwards run in the loop below
           break;
         }
-        if (s != 0) {
+        float t[3];
 #pragma unroll
-          for (int i = 0; i < 3; i++) {
-            g[i] += (int)wc[i];
-            wc[i] = wc[i] - floorf(wc[i]);
-          }
-          s = 0;
+        for (int i = 0; i < 3; i++) t[i] = ((float)(pv[i] << s) - wc[i]) * inv[i];
+        m = t[0] < t[1] ? (t[0] < t[2] ? 0 : 2) : (t[1] < t[2] ? 1 : 2);
+        float tm = m == 0 ? t[0] : (m == 1 ? t[1] : t[2]);
+        int sm = m == 0 ? sgn[0] : (m == 1 ? sgn[1] : sgn[2]);
+        int pm = m == 0 ? pv[0] : (m == 1 ? pv[1] : pv[2]);
+        float am = (float)((1 - pm) << s) * 0.999f;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          float nw = wc[i] + d[i] * tm;
+          wc[i] = (i == m) ? am : nw;
+          g[i] += (i == m) ? sm * (1 << s) : 0;
         }
-      } else {
-        if (s != 3) {
-#pragma unroll
-          for (int i = 0; i < 3; i++) {
-            int lo = g[i] & 7;
-            wc[i] = wc[i] + (float)lo;
-            g[i] -= lo;
-          }
-          s = 3;
-        }
-        fast = true;  // this iteration's step onwards run in the loop below
-        break;
+        step++;
       }
-      float t[3];
-#pragma unroll
-      for (int i = 0; i < 3; i++) t[i] = ((float)(pv[i] << s) - wc[i]) * inv[i];
-      m = t[0] < t[1] ? (t[0] < t[2] ? 0 : 2) : (t[1] < t[2] ? 1 : 2);
-      float tm = m == 0 ? t[0] : (m == 1 ? t[1] : t[2]);
-      int sm = m == 0 ? sgn[0] : (m == 1 ? sgn[1] : sgn[2]);
-      int pm = m == 0 ? pv[0] : (m == 1 ? pv[1] : pv[2]);
-      float am = (float)((1 - pm) << s) * 0.999f;
-#pragma unroll
-      for (int i = 0; i < 3; i++) {
-        float nw = wc[i] + d[i] * tm;
-        wc[i] = (i == m) ? am : nw;
-        g[i] += (i == m) ? sm * (1 << s) : 0;
-      }
-      step++;
     }
     if (!fast) break;
     // empty block at the block level: the fast loop takes the next steps
@@ -1131,13 +1164,30 @@ __device__ bool trace_brickmap_fast(const BrickmapWorld& w, float ox, float oy, 
     s0.c = morton3((uint32_t)g[0] >> 3, (uint32_t)g[1] >> 3, (uint32_t)g[2] >> 3) ^ flip;
     const uint32_t mx = in_vgpr(w.mx), my = in_vgpr(w.my), mz = in_vgpr(w.mz);
     int s0steps = step;
+    int lim = max_steps;
+    if constexpr (kSeg) {
+      if (first_run && s0steps + VT_SKIP_STEPS + 1 < max_steps) lim = s0steps + VT_SKIP_STEPS + 1;
+    }
+    first_run = false;
+    bool blocked;
     if constexpr (kPositive)
-      dense_skip_positive<VT_SKIP_STEPS>(d[0], d[1], d[2], inv[0], inv[1], inv[2], mx, my, mz, s0,
-                                         s0steps, max_steps, occ);
+      blocked = dense_skip_positive<VT_SKIP_STEPS>(d[0], d[1], d[2], inv[0], inv[1], inv[2], mx,
+                                                   my, mz, s0, s0steps, lim, occ);
     else
-      dense_skip_empty<VT_SKIP_STEPS>(P3[0], P3[1], P3[2], A3[0], A3[1], A3[2], d[0], d[1], d[2],
-                                      inv[0], inv[1], inv[2], mx, my, mz, flip, s0, s0steps,
-                                      max_steps, occ);
+      blocked = dense_skip_empty<VT_SKIP_STEPS>(P3[0], P3[1], P3[2], A3[0], A3[1], A3[2], d[0],
+                                                d[1], d[2], inv[0], inv[1], inv[2], mx, my, mz,
+                                                flip, s0, s0steps, lim, occ);
+    if constexpr (kSeg) {
+      if (!blocked && lim < max_steps) {
+        park->wx = s0.wx;
+        park->wy = s0.wy;
+        park->wz = s0.wz;
+        park->c = s0.c;
+        park->steps = s0steps;
+        h.steps = -1;
+        return false;
+      }
+    }
     float fw[3];
     uint32_t fc, fM, fa, fv;
     int sF;
