@@ -1,5 +1,5 @@
 """CPU restatement of the reference's host-side world map -- TEST
-INFRASTRUCTURE ONLY (the checker for vt_world_set_voxels / get_voxels).
+INFRASTRUCTURE ONLY (the oracle for vt_world_set_voxels / get_voxels).
 
 VoxelBrickmap (src/engine/voxel.zig:25-82) over GpuBlockAllocator
 (src/engine/graphics/gpu_block_allocator.zig:4-41): a chunk table of

@@ -1,7 +1,7 @@
 /*
  * vt_oracle.c -- CPU restatement of the reference voxel ray-traversal path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker and the timed CPU baseline); the
+ * TEST INFRASTRUCTURE ONLY (the test oracle and the timed CPU baseline); the
  * product library never links or calls it.  Build: oracle/Makefile
  * (gcc -O2 -ffp-contract=off; x86-64 SSE arithmetic, so every binary32
  * operation rounds to nearest-even exactly once).

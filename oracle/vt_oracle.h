@@ -2,8 +2,8 @@
  * vt_oracle.h -- CPU restatement of the reference voxel ray-traversal path.
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load this library, and only as the
- * checker (or the timed CPU baseline); the product library never links it.
+ * cpu_baseline leg of bench.py may load this library, and only as the test
+ * oracle (or the timed CPU baseline); the product library never links it.
  *
  * It restates, in plain C99 on IEEE-754 binary32 (compiled with
  * -ffp-contract=off, so every multiply and add rounds separately):

@@ -1,7 +1,7 @@
 /*
  * vt_procgen_oracle.c -- CPU restatement of the reference world generator.
  *
- * TEST INFRASTRUCTURE ONLY (the checker and the timed CPU baseline); the
+ * TEST INFRASTRUCTURE ONLY (the test oracle and the timed CPU baseline); the
  * product library never links or calls it.  Built into liboracle.so by
  * oracle/Makefile (gcc -O2 -ffp-contract=off: every binary32 operation
  * rounds to nearest-even once).

@@ -1,5 +1,5 @@
 """ctypes binding of oracle/liboracle.so -- the CPU restatement used as the
-checker and as the CPU baseline.  Test infrastructure only."""
+test oracle and as the CPU baseline.  Test infrastructure only."""
 from __future__ import annotations
 
 import ctypes as C

@@ -150,7 +150,7 @@ def parse_vox(data: bytes):
 
 
 # asset survey: (models, min voxels, max voxels) per file
-SURVEY = {"models": (6, 385, 457), "grass": (1, 49, 49), "rock": (1, 176, 176),
+ASSET_COUNTS = {"models": (6, 385, 457), "grass": (1, 49, 49), "rock": (1, 176, 176),
           "flower": (1, 46, 46), "water": (1, 384, 384), "tree": (4, 229, 342),
           "leaves": (3, 331, 331), "dirt": (4, 443, 457), "sand": (4, 443, 457)}
 
@@ -165,8 +165,8 @@ def test_reference_assets_load_order_and_invariants():
         data = open(os.path.join(ASSETS, name + ".vox"), "rb").read()
         models, pal = parse_vox(data)
         assert vox.vox_info(data)[0] == len(models)
-        if name in SURVEY:
-            n, lo, hi = SURVEY[name]
+        if name in ASSET_COUNTS:
+            n, lo, hi = ASSET_COUNTS[name]
             assert len(models) == n and all(lo <= len(v) <= hi for _, v in models), name
         slots[name] = a.load(data)
         colours = set((pal[:, 0].astype(np.uint32) | (pal[:, 1].astype(np.uint32) << 8) |
