@@ -201,3 +201,28 @@ def test_edge_packed_shadow_rays(ctx):
             compare(g, o, stats=False, label=f"packed brickmap {wname} {cname} sun={sun}")
         w.close()
         wb.close()
+
+
+def test_edge_packed_present_fused(ctx):
+    """The fused present pass on the packed path (1280x720, shadow cap 512):
+    shaded and display images equal to a plain render followed by
+    vt_present, on the dense and the reference layout."""
+    import torch
+    W, H = 1280, 720
+    grid = WORLDS["sparse64"]
+    pal = S.palette()
+    dim, chunks, pool, atlas = BM.from_dense(grid, pal)
+    worlds = [vt.World.dense(ctx, grid, pal), vt.World.brickmap(ctx, dim, chunks, pool, atlas)]
+    cam = cameras(64)[1][1]
+    for w, cap in zip(worlds, (512, 4096)):
+        f = vt.Frame.alloc(W, H, display=True)
+        vt.render(ctx, w, cam, f, vt.params(W, H, cap, cap, S.SUN, vt.VT_FLAG_PRESENT))
+        f2 = vt.Frame.alloc(W, H)
+        vt.render(ctx, w, cam, f2, vt.params(W, H, cap, cap, S.SUN))
+        out = torch.empty_like(f2.shaded)
+        disp = torch.empty((H, W), dtype=torch.int32, device="cuda")
+        vt.present(ctx, f2.shaded, out, disp)
+        torch.cuda.synchronize()
+        assert torch.equal(f.hits, f2.hits)
+        assert torch.equal(out, f.shaded) and torch.equal(disp, f.display)
+        w.close()
