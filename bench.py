@@ -3,7 +3,9 @@
 Workload (default): 512^3 uint8 grid (fBm terrain scaled x4 in x/z plus 4,000
 16^3 noise sprites), one 1920x1080 view, one primary ray per pixel plus one
 shadow ray per primary hit toward the sun (0.4, 0.8, 0.3), max 2048 steps.
-A step is one render of that frame with the world already resident in HBM.
+A step is one render of that frame with the world already resident in HBM;
+by default 100 untimed frames (the GPU's clocks settle within the first ~50)
+precede the 200 timed ones.
 
 Prints ONE JSON line.  value = (primary + shadow rays) / kernel time, timed
 with HIP events on the stream the kernel is launched on.  roofline.achieved
@@ -686,8 +688,13 @@ def bench_copy(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: 200 for single frames, 20 for the batched and "
+                         "differentiable workloads, 10 for procgen)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps first (default: 100 for single frames -- the GPU's "
+                         "clocks settle within the first ~50 frames of 0.8 ms --, 3 for the "
+                         "others)")
     ap.add_argument("--traversal", choices=["flat", "bitmask"], default="flat")
     ap.add_argument("--workload",
                     choices=["frame", "batched", "diff", "diff_batched", "brickmap", "procgen",
@@ -702,6 +709,11 @@ def main():
                          "copy: 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
+    frame_like = args.workload in ("frame", "brickmap", "engine")
+    if args.steps is None:
+        args.steps = 200 if frame_like else (10 if args.workload == "procgen" else 20)
+    if args.warmup is None:
+        args.warmup = 100 if frame_like else (2 if args.workload == "procgen" else 3)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(rays are independent: a multi-GPU run would shard views/tiles with no exchange)",
