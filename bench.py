@@ -693,8 +693,8 @@ def main():
                          "differentiable workloads, 10 for procgen)")
     ap.add_argument("--warmup", type=int, default=None,
                     help="untimed steps first (default: 100 for single frames -- the GPU's "
-                         "clocks settle within the first ~50 frames of 0.8 ms --, 3 for the "
-                         "others)")
+                         "clocks settle within the first ~50 frames of 0.8 ms --, 2 for "
+                         "procgen, 20 for the others)")
     ap.add_argument("--traversal", choices=["flat", "bitmask"], default="flat")
     ap.add_argument("--workload",
                     choices=["frame", "batched", "diff", "diff_batched", "brickmap", "procgen",
@@ -713,7 +713,7 @@ def main():
     if args.steps is None:
         args.steps = 200 if frame_like else (10 if args.workload == "procgen" else 20)
     if args.warmup is None:
-        args.warmup = 100 if frame_like else (2 if args.workload == "procgen" else 3)
+        args.warmup = 100 if frame_like else (2 if args.workload == "procgen" else 20)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(rays are independent: a multi-GPU run would shard views/tiles with no exchange)",
