@@ -36,7 +36,6 @@
 //   5. The block-occupancy bytes of the fast traversal are rebuilt.
 #include <algorithm>
 #include <cstring>
-#include <new>
 #include <unordered_map>
 #include <vector>
 
