@@ -43,12 +43,15 @@
 
 namespace vt {
 
-// amdgpu_waves_per_eu(8): at most 64 VGPRs, so eight waves per SIMD hide the
-// gathers of the traversal loop (the few spills it costs lie outside it).
+// amdgpu_waves_per_eu: the reference composite at most 64 VGPRs (eight waves
+// per SIMD hide the gathers of the traversal loop; the spills it costs lie
+// outside it; 1080p 0.785 vs 0.804 ms at seven), the Lambert modes at most
+// 72 (seven waves: C3's Lambert forward 0.117 vs 0.120 ms per view).
 // kPack: the block packs its long shadow rays (reference composite, flat
 // lookup, no statistics); the launcher picks it for frames that fill the GPU.
 template <class L, bool kStats, int kShade, bool kPack = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_render_dense(
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(kShade == kShadeRef ? 8 : 7))) void k_render_dense(
     DenseWorld w, ViewConsts v, ShadeConsts sc, FrameOut out, uint32_t log2_tw,
     uint32_t blocks_x) {
   const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -90,12 +93,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   }
 }
 
-// Same 8-wave budget as k_render_dense (0.97 vs 1.02 ms at 90 VGPRs / 5 waves).
+// Seven waves per SIMD (at most 72 VGPRs): the engine's own frame 0.373 vs
+// 0.460 ms at eight (fewer spills around its many literal-loop iterations),
+// the C1 world in this layout 0.823 vs 0.820 ms; five waves (90 VGPRs) were
+// slower on both.
 // kPack: the block packs its long shadow rays after a first segment and
 // finishes them 64 to a wave, as k_render_dense does (worlds with occupancy
 // bytes, no statistics).
 template <bool kStats, bool kPack = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_render_brickmap(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_render_brickmap(
     BrickmapWorld w, ViewConsts v, ShadeConsts sc, FrameOut out, uint32_t log2_tw,
     uint32_t blocks_x) {
   const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
