@@ -425,7 +425,9 @@ int vt_render(vt_ctx* ctx, const vt_world* world, const vt_camera* camera,
   const vt::ViewConsts v = vt::view_consts(camera, p.width, p.height);
   const vt::ShadeConsts sc = vt::shade_consts(p);
   const vt::FrameOut o = vt::frame_out(frame);
-  vt::TileShape tile{3};
+  // 16x4-pixel tiles per wave (blocks of 32x8): 1080p 0.766 vs 0.769 ms for
+  // 8x8 and 0.780 for 32x2; the engine's own frame 0.354 vs 0.365 ms.
+  vt::TileShape tile{4};
   hipError_t e;
   if (world->kind == WorldKind::Dense)
     e = vt::launch_render_dense(vt::dense_view(world), p.traversal, v, sc, o, tile,

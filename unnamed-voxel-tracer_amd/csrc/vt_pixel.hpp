@@ -13,10 +13,10 @@ namespace vt {
 // costs a few steps per ray, a band of near terrain hundreds).  So the image
 // blocks, in row-major order, are cut into strips of kXcdStrip blocks dealt
 // to the XCDs in turn: XCD x traces strips x, x + 8, x + 16, ...  Ids past the
-// last full round of 8 strips map to themselves.  Strips of 4 blocks
-// (16x64 pixels at 8x8 tiles): at 1080p 0.772 ms against 0.775 (1), 0.773
-// (2), 0.783 (8) and 0.792 (16); the engine's own frame and the reference
-// layout gain 2-3% the same way (clocks settled, one box).
+// last full round of 8 strips map to themselves.  Strips of 4 blocks (at
+// 8x8 tiles, measured: at 1080p 0.772 ms against 0.775 for 1, 0.773 for 2,
+// 0.783 for 8 and 0.792 for 16; the engine's own frame and the reference
+// layout gain 2-3% the same way; clocks settled, one box).
 constexpr uint32_t kXcdStrip = 4u;
 
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t n) {
