@@ -425,9 +425,12 @@ int vt_render(vt_ctx* ctx, const vt_world* world, const vt_camera* camera,
   const vt::ViewConsts v = vt::view_consts(camera, p.width, p.height);
   const vt::ShadeConsts sc = vt::shade_consts(p);
   const vt::FrameOut o = vt::frame_out(frame);
-  // 16x4-pixel tiles per wave (blocks of 32x8): 1080p 0.766 vs 0.769 ms for
-  // 8x8 and 0.780 for 32x2; the engine's own frame 0.354 vs 0.365 ms.
-  vt::TileShape tile{4};
+  // Tiles per wave: 16x4 pixels for frames that fill the GPU (at least 8
+  // blocks of 4 waves per CU: 1080p 0.766 vs 0.769 ms for 8x8 and 0.780 for
+  // 32x2; the engine's own frame 0.354 vs 0.365 ms), 8x8 below that (C3's
+  // 512x512 views: 0.1165 vs 0.1185 ms per view).
+  const uint32_t blocks_8x8 = ((p.width + 15u) / 16u) * ((p.height + 15u) / 16u);
+  vt::TileShape tile{blocks_8x8 >= 8u * (uint32_t)ctx->compute_units ? 4u : 3u};
   hipError_t e;
   if (world->kind == WorldKind::Dense)
     e = vt::launch_render_dense(vt::dense_view(world), p.traversal, v, sc, o, tile,
