@@ -24,6 +24,7 @@ sys.path.insert(0, os.path.join(ROOT, "unnamed-voxel-tracer_amd"))
 
 METRIC = ("Mrays/s primary+shadow, 1080p/512³ grid; fwd+bwd ms per view vs plain kernel")
 HBM_PEAK_GBS = 8000.0
+TRAVERSALS = ["flat", "bitmask", "bitmask8"]  # VT_TRAVERSAL_FLAT, _BITMASK, _BITMASK8
 OUT_BYTES_PER_PIXEL = 16 + 12  # RGBA32F shaded + 12-byte hit record
 
 
@@ -31,10 +32,10 @@ def algorithmic_bytes(stats, traversal, npix):
     """Bytes one launch must move: the traversal's gathers plus the outputs.
 
     flat:    1 byte per executed traceMap step (the voxel gather);
-    bitmask: 8 B per 16^3-region word + 8 B per 4^3-brick word fetched,
+    bitmask, bitmask8: 8 B per region word + 8 B per leaf word fetched,
              + 1 B material per primary hit.
     """
-    if traversal == "bitmask":
+    if traversal in ("bitmask", "bitmask8"):
         b = 8 * stats["l1_fetches"] + 8 * stats["l0_fetches"] + stats["leaf_reads"]
     else:
         b = stats["primary_steps"] + stats["shadow_steps"]
@@ -42,17 +43,33 @@ def algorithmic_bytes(stats, traversal, npix):
 
 
 def cpu_baseline(grid, pal, cam, W, H, max_steps, sun):
+    """The oracle (CPU restatement) on this host's cores, one whole frame.
+    Returns (baseline record, the oracle's outputs) -- the outputs serve as
+    the checker of the GPU frame."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi as O
-    threads = min(len(os.sched_getaffinity(0)), 16)
+    threads = O.cpu_threads()
     world = O.DenseWorld(grid, pal)
     t0 = time.perf_counter()
     out = O.render(world, cam, W, H, max_steps, max_steps, sun, threads=threads, gbuffer=False)
     dt = time.perf_counter() - t0
     rays = W * H + int(out["stats"][2])
-    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"one full {W}x{H} frame of the same workload, {threads} threads, "
-                      f"{dt:.2f} s wall"}
+    return ({"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+             "sample": f"one full {W}x{H} frame of the same workload, {threads} threads, "
+                       f"{dt:.2f} s wall"}, out)
+
+
+def frame_parity(np, hits, shaded, o):
+    """The timed kernel's last frame against the oracle's: hit records (bit
+    for bit), shaded RGB (max-abs), and how many pixels' rays met a
+    crossing-time tie margin under 1e-6 / 1e-4 (reproduced exactly by the
+    tie rule, so none is exempted)."""
+    bad = int(np.any(hits != o["hits"], axis=-1).sum())
+    err = float(np.max(np.abs(shaded - o["shaded"])))
+    m = o["tie_margin"]
+    return {"checked": "whole frame vs the oracle", "hit_records_differ": bad,
+            "shaded_max_abs": err, "tie_margin_lt_1e-6": int((m < 1e-6).sum()),
+            "tie_margin_lt_1e-4": int((m < 1e-4).sum()), "exempted": 0}
 
 
 def timed(torch, stream, fn, steps, warmup):
@@ -141,7 +158,7 @@ def bench_batched(args):
             "dda_steps": st["primary_steps"] + st["shadow_steps"],
             "tuning": {"tile": {3: "8x8", 4: "16x4", 5: "32x2"}[best.tile],
                        "chunk": best.chunk,
-                       "traversal": ["flat", "bitmask"][best.traversal],
+                       "traversal": TRAVERSALS[best.traversal],
                        "blocks_per_cu": best.blocks_per_cu},
             "autotune_eager_ms": round(tune_ms, 4),
             "plain_ms_all_views": round(plain_ms, 4),
@@ -232,44 +249,11 @@ def bench_brickmap(args):
     ctx.close()
 
 
-def _part1by2(t):
-    t = t & 0x3FF
-    t = (t | (t << 16)) & 0xFF0000FF
-    t = (t | (t << 8)) & 0x0300F00F
-    t = (t | (t << 4)) & 0x030C30C3
-    return (t | (t << 2)) & 0x09249249
-
-
-def _ground_truth_albedo(torch, grid, pal, compact, seed):
-    """Material base colour + N(0, 0.05) per voxel, float32 RGB.
-
-    dense:   dim^3 x 3, indexed by voxel id x + dim*(y + dim*z);
-    compact: n_occupied x 3, occupied voxels in Morton order (the order of the
-             library's occupancy prefix), built on the GPU.
-    """
-    base = torch.tensor([[(int(p) >> s) & 0xFF for s in (0, 8, 16)] for p in pal],
-                        dtype=torch.float32, device="cuda") / 255.0
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(seed)
-    g = torch.from_numpy(grid.reshape(-1)).cuda()
-    if compact:
-        dim = grid.shape[0]
-        lg = dim.bit_length() - 1
-        occ = torch.nonzero(g).squeeze(1)
-        code = (_part1by2(occ & (dim - 1)) | (_part1by2((occ >> lg) & (dim - 1)) << 1) |
-                (_part1by2(occ >> (2 * lg)) << 2))
-        order = torch.argsort(code)
-        mats = g[occ[order]].long()
-        del occ, code, order
-    else:
-        mats = g.long()
-    del g
-    a = base[mats]
-    a += 0.05 * torch.randn(a.shape, generator=gen, device="cuda")
-    return a.reshape(-1).contiguous()
-
-
 def bench_diff(args, scene):
+    print(json.dumps(measure_diff(args, scene)), flush=True)
+
+
+def measure_diff(args, scene):
     """Inverse-rendering step: Lambert x shadow forward over every view, then
     dL/dalbedo and dL/dsun of L = sum ||I - I_gt||^2 with the hit structure
     held fixed.  Albedo starts at 0.5; targets are renders of the ground truth
@@ -306,7 +290,7 @@ def bench_diff(args, scene):
     if batched:
         n_occ = vt.build_compact_index(ctx, world)
         flags |= vt.VT_FLAG_ALBEDO_COMPACT
-    gt = _ground_truth_albedo(torch, grid, pal, batched, 0x5EED)
+    gt = S.ground_truth_albedo(grid, pal, batched, 0x5EED)
     if batched:
         assert gt.numel() == 3 * n_occ, (gt.numel(), n_occ)
     del grid
@@ -486,12 +470,12 @@ def bench_diff(args, scene):
                      "unit": "GB/s", "frac": round(fwd_gbs / HBM_PEAK_GBS, 5), "traffic": None},
         "cpu_baseline": None,
     }
-    print(json.dumps(line), flush=True)
     if batched:
         for b in (plain_b, fwd_b, rec_b):
             b.close()
     world.close()
     ctx.close()
+    return line
 
 
 AMBIENT_DIFF = 0.2
@@ -695,7 +679,7 @@ def main():
                     help="untimed steps first (default: 100 for single frames -- the GPU's "
                          "clocks settle within the first ~50 frames of 0.8 ms --, 2 for "
                          "procgen, 20 for the others)")
-    ap.add_argument("--traversal", choices=["flat", "bitmask"], default="flat")
+    ap.add_argument("--traversal", choices=TRAVERSALS, default="flat")
     ap.add_argument("--workload",
                     choices=["frame", "batched", "diff", "diff_batched", "brickmap", "procgen",
                              "engine", "copy"],
@@ -708,6 +692,8 @@ def main():
                          "engine: the engine's own frame (its world, camera and step caps); "
                          "copy: 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fwd-bwd", action="store_true",
+                    help="skip the C3 fwd+bwd block of the default line")
     args = ap.parse_args()
     frame_like = args.workload in ("frame", "brickmap", "engine")
     if args.steps is None:
@@ -740,6 +726,23 @@ def main():
     import vt
     from vt import scenes as S
 
+    # the metric's second clause, measured in the same run: C3 fwd+bwd per
+    # view against the plain forward (its own world and views)
+    fwd_bwd = None
+    if not args.no_fwd_bwd:
+        d = measure_diff(args, "c3")
+        c = d["config"]
+        fwd_bwd = {
+            "workload": c["workload"],
+            "plain_fwd_ms_per_view": c["plain_fwd_ms_per_view"],
+            "stored_fwd_bwd_ms_per_view": c["stored_fwd_bwd_ms_per_view"],
+            "recompute_fwd_bwd_ms_per_view": c["recompute_fwd_bwd_ms_per_view"],
+            "backward_ms_per_view": c["backward_ms_per_view"],
+            "stored_over_plain": c["fwd_bwd_over_plain"],
+            "recompute_over_plain": c["recompute_over_plain"],
+            "modes_agree": c["modes_agree"],
+        }
+
     print("[bench] generating 512^3 world", file=sys.stderr, flush=True)
     grid = S.generate(S.spec("c1"))
     pal = S.palette()
@@ -747,7 +750,7 @@ def main():
     max_steps = S.MAX_STEPS["c1"]
     ctx = vt.Context(0)
     world = vt.World.dense(ctx, grid, pal)
-    trav = vt.VT_TRAVERSAL_BITMASK if args.traversal == "bitmask" else vt.VT_TRAVERSAL_FLAT
+    trav = TRAVERSALS.index(args.traversal)
 
     # one counted launch (statistics kernel variant) for rays and bytes
     fs = vt.Frame.alloc(W, H, stats=True)
@@ -784,10 +787,12 @@ def main():
     achieved = abytes / (ms * 1e-3) / 1e9
     steps_total = stats["primary_steps"] + stats["shadow_steps"]
 
-    cpu = None
+    cpu = parity = None
     if not args.no_cpu:
         print("[bench] cpu baseline (oracle, one frame)", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(grid, pal, cam, W, H, max_steps, S.SUN)
+        cpu, o = cpu_baseline(grid, pal, cam, W, H, max_steps, S.SUN)
+        parity = frame_parity(np, f.hits.cpu().numpy().view(np.uint32), f.shaded.cpu().numpy(), o)
+        print(f"[bench] parity {parity}", file=sys.stderr, flush=True)
 
     line = {
         "metric": METRIC,
@@ -821,6 +826,8 @@ def main():
             "traffic": measured_traffic(f"k_render_dense/{args.traversal}/512cube_1080p"),
         },
         "cpu_baseline": cpu,
+        "parity": parity,
+        "fwd_bwd": fwd_bwd,
     }
     print(json.dumps(line), flush=True)
     world.close()

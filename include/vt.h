@@ -68,7 +68,10 @@ typedef struct vt_camera {
                                       /* stored through rgba8 clamp+round,    */
                                       /* as gbuffer.zig:10-13 declares them   */
 #define VT_FLAG_LAMBERT (1u << 1)     /* shaded = albedo*(ambient+max(n.l,0)*vis) */
-                                      /* instead of the blit composite        */
+                                      /* instead of the blit composite; l is  */
+                                      /* sun/|sun|, so a sun whose binary32   */
+                                      /* squared length is 0 or overflows is  */
+                                      /* VT_ERR_INVALID_ARGUMENT              */
 #define VT_FLAG_ALBEDO_COMPACT (1u << 2) /* albedo / its gradient hold occupied */
                                       /* voxels only, in the order of the     */
                                       /* world's compact index                */
@@ -83,8 +86,11 @@ typedef struct vt_camera {
                                       /* when set, gets it as RGBA8           */
 
 /* traversal storage variants for dense worlds (identical hits by contract) */
-#define VT_TRAVERSAL_FLAT 0u    /* one uint8 gather per executed step        */
-#define VT_TRAVERSAL_BITMASK 1u /* 2-level uint64 occupancy (4^3 / 16^3)     */
+#define VT_TRAVERSAL_FLAT 0u     /* one uint8 gather per executed step       */
+#define VT_TRAVERSAL_BITMASK 1u  /* 2-level uint64 occupancy: 4^3 bricks,    */
+                                 /* one bit each in 16^3-region words        */
+#define VT_TRAVERSAL_BITMASK8 2u /* 2-level uint64 occupancy: 8^3 bricks,    */
+                                 /* one bit each in 32^3-region words        */
 
 typedef struct vt_render_params {
   uint32_t width, height;    /* image size in pixels                        */
@@ -245,6 +251,9 @@ int vt_world_read_brickmap(vt_ctx* ctx, const vt_world* world, uint32_t* chunks,
 int vt_world_procgen(vt_ctx* ctx, vt_world* world, float offset_x, float offset_y,
                      void* stream);
 
+/* vt_world_get_voxels, vt_pick and vt_world_read_brickmap allocate no device
+ * memory: they stage through a buffer each vt_ctx allocates once.  A failed
+ * vt_world_set_voxels or vt_world_procgen leaves the world as it was.      */
 /* terrain_edit.comp.glsl:11-24: the camera's centre ray (rayUV = 0) through
  * the AABB entry and traceMap with max_steps (64 in the shader).  place /
  * remove are the block cells the shader's (disabled) edit body would write:
@@ -335,7 +344,8 @@ typedef struct vt_view {
 typedef struct vt_tuning {
   uint32_t tile;          /* VT_TILE_*                                      */
   uint32_t chunk;         /* tiles a wave takes per queue grab, 1..64       */
-  uint32_t traversal;     /* VT_TRAVERSAL_* (overrides params->traversal)   */
+  uint32_t traversal;     /* VT_TRAVERSAL_* (overrides params->traversal):  */
+                          /* the storage walked and its brick size         */
   uint32_t blocks_per_cu; /* persistent 256-thread blocks per CU, 1..16     */
 } vt_tuning;
 
@@ -355,8 +365,13 @@ int vt_batch_capture_graph(vt_batch* batch);
 int vt_batch_graph_launch(vt_batch* batch, void* stream);
 int vt_batch_destroy(vt_batch* batch);
 /* The autotuner: times every tile shape x chunk {1,2,4,8} x traversal
- * (flat, bitmask) on these views (`reps` timed launches each after one
- * warm-up) and returns the fastest tuning and its mean launch time. */
+ * (flat; bitmask with 4^3 or with 8^3 bricks) on these views (`reps` timed launches each after one
+ * warm-up) and returns the fastest tuning and its mean launch time.  Every
+ * timed launch writes the views' frames (their contents are those of a
+ * render when it returns); VT_FLAG_GRAD_L2 is rejected
+ * (VT_ERR_INVALID_ARGUMENT), since each launch would add into the caller's
+ * gradient accumulators -- tune with VT_FLAG_LAMBERT, the same kernel
+ * shape without the accumulation. */
 int vt_batch_autotune(vt_ctx* ctx, const vt_world* world, const vt_view* views,
                       uint32_t n_views, const vt_render_params* params, int reps,
                       vt_tuning* best, float* best_ms);

@@ -384,7 +384,7 @@ static void render_pixel(job* J, uint32_t px, uint32_t py, uint64_t st[8]) {
   or_entry_origin(o, d, w->dim, e);
   or_hit h;
   int hit = or_trace_map(w, e, d, p->max_steps_primary, &h, NULL, 0);
-  const int bitmask = p->traversal == 1u;
+  const int bitmask = p->traversal == 1u || p->traversal == 2u; /* 4^3 or 8^3 bricks */
   st[0] += (uint64_t)h.steps;
   /* bitmask traversal: the material byte is read once at a primary hit (its
    * word fetches, stats 4 and 5, depend on the kernel's cache and are counted

@@ -109,7 +109,8 @@ typedef struct or_params {
   int32_t max_steps_primary, max_steps_shadow;
   float sun[3];
   uint32_t flags;     /* bit0 GBUF_UNORM8, bit1 LAMBERT, bit4 PRESENT */
-  uint32_t traversal; /* 0 flat, 1 bitmask (dense worlds: memory counters only) */
+  uint32_t traversal; /* 0 flat, 1 / 2 bitmask with 4^3 / 8^3 bricks (dense worlds:
+                         memory counters only) */
   float ambient;
   const float* albedo; /* LAMBERT: dim^3*3 floats */
 } or_params;

@@ -158,9 +158,15 @@ def trace(world, o, d, max_steps, path_cap=0):
     return h, path[: min(h.steps, path_cap)] if path_cap else None
 
 
+def cpu_threads():
+    """Threads for the oracle: this process's CPU share, at most 16."""
+    return max(1, min(len(os.sched_getaffinity(0)), 16))
+
+
 def render(world, cam, width, height, max_primary, max_shadow, sun, flags=0, traversal=0,
-           ambient=0.0, albedo=None, rows=None, threads=8, gbuffer=True, display=False):
+           ambient=0.0, albedo=None, rows=None, threads=None, gbuffer=True, display=False):
     """Full (or row-range) oracle render; returns a dict of numpy arrays."""
+    threads = cpu_threads() if threads is None else threads
     p = OrParams(width, height, max_primary, max_shadow, fvec(sun), flags, traversal,
                  float(ambient), None if albedo is None else albedo.ctypes.data)
     out = {

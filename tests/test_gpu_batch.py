@@ -62,7 +62,7 @@ def assert_same(frames, ref, label):
 
 @pytest.mark.parametrize("tile", [vt.VT_TILE_8X8, vt.VT_TILE_16X4, vt.VT_TILE_32X2])
 @pytest.mark.parametrize("chunk", [1, 4])
-@pytest.mark.parametrize("traversal", [vt.VT_TRAVERSAL_FLAT, vt.VT_TRAVERSAL_BITMASK])
+@pytest.mark.parametrize("traversal", [vt.VT_TRAVERSAL_FLAT, vt.VT_TRAVERSAL_BITMASK, vt.VT_TRAVERSAL_BITMASK8])
 def test_batch_matches_per_view_render(ctx, setup, tile, chunk, traversal):
     import torch
     world, cams, prm, ref = setup
@@ -114,8 +114,31 @@ def test_autotune_picks_a_valid_tuning(ctx, setup):
     frames = fresh_frames()
     best, ms = vt.autotune(ctx, world, list(zip(cams, frames)), prm, reps=1)
     assert best.tile in (3, 4, 5) and best.chunk in (1, 2, 4, 8)
-    assert best.traversal in (0, 1) and best.blocks_per_cu == 8
+    assert best.traversal in (0, 1, 2) and best.blocks_per_cu == 8
     assert 0.0 < ms < 1000.0
+    # the frames hold a render when it returns
+    assert_same(frames, ref, "after autotune")
+
+
+def test_autotune_rejects_gradient_accumulation(ctx, setup):
+    """VT_FLAG_GRAD_L2 would add into the caller's accumulators on every
+    timed launch: refused before any launch (the accumulators stay zero)."""
+    import torch
+    world, cams, _, _ = setup
+    frames = fresh_frames()
+    for f in frames:
+        f.target = torch.zeros_like(f.shaded)
+    alb = torch.full((128 ** 3 * 3,), 0.5, device="cuda")
+    ga = torch.zeros_like(alb)
+    gs = torch.zeros(3, device="cuda")
+    loss = torch.zeros(1, device="cuda")
+    prm = vt.params(0, 0, 512, 512, S.SUN, vt.VT_FLAG_LAMBERT | vt.VT_FLAG_GRAD_L2, 0, 0.2, alb,
+                    ga, gs, loss)
+    with pytest.raises(vt.VtError) as e:
+        vt.autotune(ctx, world, list(zip(cams, frames)), prm, reps=1)
+    assert e.value.status == vt._native.VT_ERR_INVALID_ARGUMENT
+    torch.cuda.synchronize()
+    assert not ga.any() and not gs.any() and not loss.any()
 
 
 def test_batch_present_matches_per_view(ctx, setup):

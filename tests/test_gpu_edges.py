@@ -61,7 +61,7 @@ SIZES = [(1, 1), (1, 7), (13, 1), (65, 3), (40, 33)]
 
 
 @pytest.mark.parametrize("wname", list(WORLDS))
-@pytest.mark.parametrize("traversal", [vt.VT_TRAVERSAL_FLAT, vt.VT_TRAVERSAL_BITMASK])
+@pytest.mark.parametrize("traversal", [vt.VT_TRAVERSAL_FLAT, vt.VT_TRAVERSAL_BITMASK, vt.VT_TRAVERSAL_BITMASK8])
 def test_edge_worlds_cameras_sizes(ctx, wname, traversal):
     grid = WORLDS[wname]
     pal = S.palette()
@@ -84,7 +84,7 @@ def test_edge_step_caps(ctx, caps):
     w = vt.World.dense(ctx, grid, pal)
     ow = O.DenseWorld(grid, pal)
     for cname, cam in cameras(64)[1:4]:
-        for traversal in (vt.VT_TRAVERSAL_FLAT, vt.VT_TRAVERSAL_BITMASK):
+        for traversal in (vt.VT_TRAVERSAL_FLAT, vt.VT_TRAVERSAL_BITMASK, vt.VT_TRAVERSAL_BITMASK8):
             g = gpu_render(ctx, w, cam, 40, 33, caps[0], caps[1], traversal=traversal)
             o = O.render(ow, cam, 40, 33, caps[0], caps[1], S.SUN, traversal=traversal)
             compare(g, o, label=f"caps {caps} {cname} trav={traversal}")

@@ -155,7 +155,7 @@ def test_pick_brickmap_and_dense(ctx, edited_brickmap):
     wd.close()
 
 
-@pytest.mark.parametrize("traversal", [vt.VT_TRAVERSAL_FLAT, vt.VT_TRAVERSAL_BITMASK])
+@pytest.mark.parametrize("traversal", [vt.VT_TRAVERSAL_FLAT, vt.VT_TRAVERSAL_BITMASK, vt.VT_TRAVERSAL_BITMASK8])
 def test_dense_edits_render_like_edited_grid(ctx, traversal):
     import torch
     grid = S.generate(S.spec("c0"))
@@ -204,4 +204,63 @@ def test_edit_argument_errors(ctx):
         w.get_voxels([[0, 0, 99]])
     assert e.value.status == vt._native.VT_ERR_INVALID_ARGUMENT
     assert w.get_voxels([[1, 2, 3]])[0] == 0  # nothing was written by the failed calls
+    w.close()
+
+
+def test_reads_in_staging_chunks(ctx):
+    """vt_world_get_voxels stages through the context's buffer 16,384 voxels
+    at a time: a read of 3 x that plus a ragged tail equals the grid, on
+    both layouts, and a pick repeated 200 times gives one answer."""
+    grid = S.generate(S.spec("c0"))
+    pal = S.palette()
+    w = vt.World.dense(ctx, grid, pal)
+    rng = np.random.default_rng(9)
+    xyz = rng.integers(0, 128, (3 * 16384 + 123, 3)).astype(np.uint32)
+    log("dense read of 49,275 voxels")
+    assert np.array_equal(w.get_voxels(xyz), grid[xyz[:, 2], xyz[:, 1], xyz[:, 0]])
+    import brickmap_util as BU
+    dim, chunks, pool, atlas = BU.embedding_a(grid, pal)
+    wb = vt.World.brickmap(ctx, dim, chunks, pool, atlas)
+    ow = O.BrickmapWorld(dim, chunks, pool, atlas)
+    got = wb.get_voxels(xyz)
+    exp = np.array([0 if chunks[(x // 8) + (dim // 8) * ((y // 8) + (dim // 8) * (z // 8))] == 0
+                    else pool[(chunks[(x // 8) + (dim // 8) * ((y // 8) + (dim // 8) * (z // 8))]
+                               - 1) * 512 + (x % 8) + 8 * ((y % 8) + 8 * (z % 8))]
+                    for x, y, z in xyz], np.uint32)
+    assert np.array_equal(got, exp)
+    cam = vcam.look_at((64.0, 110.0, -20.0), (64.0, 40.0, 64.0), 70.0)
+    o = O.pick(ow, cam, 64)
+    for _ in range(200):
+        g = wb.pick(cam, 64)
+        assert (g.data, g.face, g.steps) == (o.data, o.face, o.steps)
+    wb.close()
+    w.close()
+
+
+def test_failed_brickmap_edit_leaves_world_unchanged(ctx, monkeypatch):
+    """A pool growth refused by the VT_POOL_LIMIT_BYTES budget fails the
+    call before anything changes: the chunk table and pool read back as
+    before, and the next successful edit allocates the next block (no two
+    chunks share a block)."""
+    atlas = atlas_fixture()
+    w = vt.World.empty_brickmap(ctx, 64, atlas)
+    w.set_voxels([[1, 1, 1]], [3 | (1 << 28)])  # one chunk, one block
+    c0, p0 = w.read_brickmap()
+    assert p0.size == 512
+    # 40 new chunks need 41 blocks: the pool would grow to 64 blocks (128 KiB)
+    xyz = np.array([[8 * (i % 8), 8 * (i // 8), 0] for i in range(1, 41)], np.uint32)
+    vals = np.full(len(xyz), 5 | (1 << 28), np.uint32)
+    monkeypatch.setenv("VT_POOL_LIMIT_BYTES", str(16 * 2048))
+    with pytest.raises(vt.VtError) as e:
+        w.set_voxels(xyz, vals)
+    assert e.value.status == vt._native.VT_ERR_OUT_OF_MEMORY
+    c1, p1 = w.read_brickmap()
+    assert np.array_equal(c0, c1) and np.array_equal(p0, p1)
+    monkeypatch.delenv("VT_POOL_LIMIT_BYTES")
+    w.set_voxels(xyz[:2], vals[:2])
+    c2, p2 = w.read_brickmap()
+    used = c2[c2 != 0]
+    assert sorted(used.tolist()) == [1, 2, 3] and p2.size == 3 * 512
+    assert np.array_equal(w.get_voxels(np.vstack([[[1, 1, 1]], xyz[:2]])),
+                          np.array([3 | (1 << 28), 5 | (1 << 28), 5 | (1 << 28)], np.uint32))
     w.close()

@@ -87,14 +87,15 @@ def test_c2_batch_and_traversals_equal_plain(ctx, c2):
             c = getattr(r, k).cpu().numpy().view(np.uint32)
             assert np.array_equal(a, c), f"c2 view {i}: batch {k} differs from vt_render"
     b.close()
-    print("[full-size] c2 flat vs bitmask on every view", flush=True)
+    print("[full-size] c2 flat vs bitmask (4^3 and 8^3 bricks) on every view", flush=True)
     for i, ((cam, W, H), r) in enumerate(zip(views, plain)):
-        fb = _render(ctx, world, cam, W, H, traversal=vt.VT_TRAVERSAL_BITMASK)
-        torch.cuda.synchronize()
-        for k in ("hits", "steps"):
-            a = getattr(fb, k).cpu().numpy().view(np.uint32)
-            c = getattr(r, k).cpu().numpy().view(np.uint32)
-            assert np.array_equal(a, c), f"c2 view {i}: bitmask {k} differs from flat"
+        for trav in (vt.VT_TRAVERSAL_BITMASK, vt.VT_TRAVERSAL_BITMASK8):
+            fb = _render(ctx, world, cam, W, H, traversal=trav)
+            torch.cuda.synchronize()
+            for k in ("hits", "steps"):
+                a = getattr(fb, k).cpu().numpy().view(np.uint32)
+                c = getattr(r, k).cpu().numpy().view(np.uint32)
+                assert np.array_equal(a, c), f"c2 view {i}: traversal {trav} {k} differs from flat"
 
 
 def test_c4_lambert_batch_compact_albedo(ctx, c2):
@@ -154,3 +155,184 @@ def test_c4_lambert_batch_compact_albedo(ctx, c2):
         ucode, first = np.unique(code, return_index=True)
         assert np.all(np.diff(rank[first]) > 0), f"c4 view {i}: compact index not increasing"
     b.close()
+
+
+# ---------------------------------------------------------------------------
+# whole views against the oracle
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("part", range(4))
+def test_c2_whole_views_vs_oracle(ctx, c2, part):
+    """Every C2 view whole (16 views per part): the timed kernel's hit
+    records, step counts and G-buffer bit-exact; the statistics kernel too
+    on the views of at most 512x512 pixels."""
+    import torch
+    grid, pal, world = c2
+    ow = O.DenseWorld(grid, pal)
+    views = S.views("c2")
+    cap = S.MAX_STEPS["c2"]
+    n6 = n4 = 0
+    for i in range(16 * part, 16 * part + 16):
+        cam, W, H = views[i]
+        g = {"fast": _dict(_render(ctx, world, cam, W, H, gbuffer=True))}
+        small = W * H <= 512 * 512
+        if small:
+            g.update(_dict(_render(ctx, world, cam, W, H, stats=True, gbuffer=True)))
+        torch.cuda.synchronize()
+        o = O.render(ow, cam, W, H, cap, cap, S.SUN)
+        m = o["tie_margin"]
+        n6 += int((m < 1e-6).sum())
+        n4 += int((m < 1e-4).sum())
+        label = f"c2 view {i} {W}x{H} whole"
+        if small:
+            compare(g, o, stats=False, label=label)
+        else:
+            compare(g["fast"], o, stats=False, label=label + " [no-stats kernel]")
+    print(f"[full-size] c2 views {16 * part}..{16 * part + 15}: tie margin < 1e-6 on {n6} "
+          f"pixels, < 1e-4 on {n4} (compared bit for bit, none exempted)", flush=True)
+
+
+# ---------------------------------------------------------------------------
+# C4: the differentiable batch at full size
+# ---------------------------------------------------------------------------
+
+AMBIENT_C4 = 0.2
+
+
+def test_c4_backward_full_size(ctx, c2):
+    """48 views (20 M pixels), compact albedo (72.7 M occupied voxels),
+    ground truth = material colour + N(0, 0.05), start 0.5: the fused
+    recompute batch (VT_FLAG_GRAD_L2) against the stored-record backward
+    (Lambert batch writing hit records, then vt_lambert_backward over all
+    records), and both against the oracle's FP64 gradient: the loss and the
+    sun gradient over every pixel, the albedo gradient of 64 random visible
+    voxels."""
+    import torch
+    grid, pal, world = c2
+    n_occ = vt.build_compact_index(ctx, world)
+    flags = vt.VT_FLAG_LAMBERT | vt.VT_FLAG_ALBEDO_COMPACT
+    views = S.views("c4")
+    cap = S.MAX_STEPS["c4"]
+    npix = sum(W * H for _, W, H in views)
+    print(f"[full-size] c4: {len(views)} views, {npix} pixels, {n_occ} occupied voxels",
+          flush=True)
+    gt = S.ground_truth_albedo(grid, pal, compact=True)
+    assert gt.numel() == 3 * n_occ
+    start = torch.full_like(gt, 0.5)
+    hits = torch.empty((npix, 3), dtype=torch.int32, device="cuda")
+    shaded = torch.empty((npix, 4), dtype=torch.float32, device="cuda")
+    target = torch.empty((npix, 4), dtype=torch.float32, device="cuda")
+
+    def frames(**bufs):
+        out, off = [], 0
+        for cam, W, H in views:
+            f = vt.Frame(W, H)
+            for k, t in bufs.items():
+                setattr(f, k, t[off:off + W * H].view(H, W, -1))
+            out.append((cam, f))
+            off += W * H
+        return out
+
+    def prm(alb, fl, grads=(None, None, None)):
+        return vt.params(0, 0, cap, cap, S.SUN, fl, 0, AMBIENT_C4, alb, *grads)
+
+    tune = vt.tuning(chunk=2)
+    print("[full-size] c4 targets: Lambert batch of the ground truth", flush=True)
+    b = vt.Batch(ctx, world, frames(shaded=target), prm(gt, flags), tune)
+    b.launch()
+    torch.cuda.synchronize()
+    b.close()
+    del gt
+    print("[full-size] c4 stored records + backward", flush=True)
+    b = vt.Batch(ctx, world, frames(shaded=shaded, hits=hits), prm(start, flags), tune)
+    b.launch()
+    b.close()
+    ga_s = torch.zeros_like(start)
+    gs_s = torch.zeros(3, dtype=torch.float32, device="cuda")
+    loss_s = torch.zeros(1, dtype=torch.float32, device="cuda")
+    vt.lambert_backward(ctx, world, hits, vt.params(1, 1, sun=S.SUN, flags=flags,
+                                                    ambient=AMBIENT_C4, albedo=start),
+                        target, vt.VT_UPSTREAM_L2_TARGET, ga_s, gs_s, loss_s)
+    print("[full-size] c4 fused recompute batch", flush=True)
+    ga_f = torch.zeros_like(start)
+    gs_f = torch.zeros(3, dtype=torch.float32, device="cuda")
+    loss_f = torch.zeros(1, dtype=torch.float32, device="cuda")
+    shaded_f = torch.empty_like(shaded)
+    b = vt.Batch(ctx, world, frames(shaded=shaded_f, target=target),
+                 prm(start, flags | vt.VT_FLAG_GRAD_L2, (ga_f, gs_f, loss_f)), tune)
+    b.launch()
+    b.close()
+    # per-entry scale of the float32 rounding error: the sum over pixels of
+    # 2 fct (|I| + |T|) -- the operands of each term's residual I - T
+    # (upstream 2 (|I| + |T|) through the upstream-gradient backward)
+    bound = torch.zeros_like(start)
+    up = (2.0 * (shaded.abs() + target.abs())).contiguous()
+    vt.lambert_backward(ctx, world, hits, vt.params(1, 1, sun=S.SUN, flags=flags,
+                                                    ambient=AMBIENT_C4, albedo=start),
+                        up, vt.VT_UPSTREAM_GRADIENT, bound)
+    torch.cuda.synchronize()
+    assert torch.equal(shaded_f, shaded), "fused and stored forwards differ"
+    diff = (ga_f - ga_s).abs()
+    worst = float((diff - (2e-5 * bound + 2e-6)).max().item())
+    print(f"[full-size] c4 loss stored {loss_s.item():.7g} fused {loss_f.item():.7g}; "
+          f"grad worst excess {worst:.3g}", flush=True)
+    assert worst <= 0.0
+    assert abs(loss_f.item() - loss_s.item()) <= 2e-5 * abs(loss_s.item())
+
+    # FP64 over every pixel: loss and sun gradient (the start albedo is 0.5
+    # everywhere, so every hit record may name one albedo slot)
+    rec = hits.cpu().numpy().view(np.uint32).copy()
+    tgt = target.cpu().numpy()
+    hit = rec[:, 0] != 0xFFFFFFFF
+    rec1 = rec.copy()
+    rec1[hit, 0] = 0
+    half = np.full(3, 0.5, np.float32)
+    da = np.zeros(3, np.float64)
+    ds = np.zeros(3, np.float64)
+    loss64 = O.lib().or_lambert_grad64(rec1.ctypes.data, npix, half.ctypes.data, O.fvec(S.SUN),
+                                       AMBIENT_C4, tgt.ctypes.data, da.ctypes.data,
+                                       ds.ctypes.data)
+    l = np.asarray(S.SUN, np.float64)
+    l /= np.linalg.norm(l)
+    normals = np.array([[-1, 0, 0], [1, 0, 0], [0, -1, 0], [0, 1, 0], [0, 0, -1], [0, 0, 1]],
+                       np.float64)
+    info = rec[hit, 2]
+    nd = normals[(info & 7).astype(np.int64) - 1] @ l
+    lit = (nd > 0) & (((info >> 3) & 1) == 0)
+    fct = AMBIENT_C4 + np.maximum(nd, 0) * (((info >> 3) & 1) == 0)
+    mag = 0.5 * fct[:, None] + np.abs(tgt[hit, :3].astype(np.float64))
+    sb = float(np.sum(2 * mag * 0.5, 1)[lit].sum() * 2 / np.linalg.norm(S.SUN))
+    for name, L_, gs in (("stored", loss_s, gs_s), ("fused", loss_f, gs_f)):
+        g = gs.cpu().numpy()
+        print(f"[full-size] c4 {name}: loss {L_.item():.7g} fp64 {loss64:.7g}; grad_sun {g} "
+              f"fp64 {ds} (term bound {sb:.3g})", flush=True)
+        assert abs(L_.item() - loss64) <= 2e-5 * abs(loss64)
+        assert np.all(np.abs(g - ds) <= 1e-5 * sb), (name, g, ds, sb)
+
+    # FP64 albedo gradient of 64 random visible voxels, through their compact
+    # index (rank of the Morton code among the occupied voxels)
+    rng = np.random.default_rng(44)
+    vis = np.unique(rec[hit, 0])
+    pick = rng.choice(vis, 64, replace=False).astype(np.int64)
+    _, codes = S.occupied_morton(grid)
+    pc = S.morton_codes(torch.from_numpy(pick).cuda(), grid.shape[0])
+    rank = torch.searchsorted(codes, pc).cpu().numpy()
+    assert np.array_equal(codes[torch.from_numpy(rank).cuda()].cpu().numpy(), pc.cpu().numpy())
+    del codes
+    slot = {int(v): j for j, v in enumerate(pick)}
+    sel = np.isin(rec[:, 0], pick)
+    sub = rec[sel].copy()
+    sub[:, 0] = [slot[int(v)] for v in sub[:, 0]]
+    sub_t = np.ascontiguousarray(tgt[sel])
+    alb = np.full(64 * 3, 0.5, np.float32)
+    da = np.zeros(64 * 3, np.float64)
+    ds2 = np.zeros(3, np.float64)
+    O.lib().or_lambert_grad64(sub.ctypes.data, sub.shape[0], alb.ctypes.data, O.fvec(S.SUN),
+                              AMBIENT_C4, sub_t.ctypes.data, da.ctypes.data, ds2.ctypes.data)
+    idx = torch.from_numpy((3 * rank[:, None] + np.arange(3)).reshape(-1)).cuda()
+    for name, ga in (("stored", ga_s), ("fused", ga_f)):
+        g = ga[idx].cpu().numpy().astype(np.float64)
+        B = bound[idx].cpu().numpy().astype(np.float64)
+        err = np.abs(g - da)
+        assert np.all(err <= 1e-5 * B + 1e-6), (name, float(np.max(err - 1e-5 * B)))
+    assert np.count_nonzero(da) >= 64

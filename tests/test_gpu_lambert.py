@@ -127,7 +127,7 @@ def oracle_shade(sc, albedo_np, sun=S.SUN):
     return out
 
 
-@pytest.mark.parametrize("traversal", [vt.VT_TRAVERSAL_FLAT, vt.VT_TRAVERSAL_BITMASK])
+@pytest.mark.parametrize("traversal", [vt.VT_TRAVERSAL_FLAT, vt.VT_TRAVERSAL_BITMASK, vt.VT_TRAVERSAL_BITMASK8])
 def test_forward_fused_records_oracle(ctx, scene, traversal):
     import torch
     sc = scene
@@ -345,3 +345,27 @@ def test_error_paths(ctx, scene):
         out = torch.empty((4, 4), device="cuda")
         vt.lambert_shade(ctx, sc["world"], sc["hits"].reshape(-1, 3)[:4].contiguous(), prm, out)
     assert e.value.status == N.VT_ERR_NULL_POINTER
+
+
+def test_zero_sun_rejected_for_lambert(ctx, scene):
+    """Lambert shading divides by |sun|: a sun of zero length (or one whose
+    squared length underflows) is VT_ERR_INVALID_ARGUMENT, not NaN output;
+    the reference composite keeps accepting it (its 0.001 substitution)."""
+    import torch
+    sc = scene
+    W, H = sc["W"], sc["H"]
+    f = vt.Frame.alloc(W, H)
+    for sun in ((0.0, 0.0, 0.0), (1e-30, 0.0, -1e-30)):
+        with pytest.raises(vt.VtError) as e:
+            vt.render(ctx, sc["world"], sc["cam"], f,
+                      vt.params(W, H, 512, 512, sun, vt.VT_FLAG_LAMBERT, 0, AMBIENT, sc["albedo"]))
+        assert e.value.status == N.VT_ERR_INVALID_ARGUMENT
+        with pytest.raises(vt.VtError) as e:
+            shade(ctx, sc, sc["albedo"], sun=sun)
+        assert e.value.status == N.VT_ERR_INVALID_ARGUMENT
+        with pytest.raises(vt.VtError) as e:
+            backward(ctx, sc, sc["albedo"], sc["target"], vt.VT_UPSTREAM_L2_TARGET, sun=sun)
+        assert e.value.status == N.VT_ERR_INVALID_ARGUMENT
+    vt.render(ctx, sc["world"], sc["cam"], f, vt.params(W, H, 512, 512, (0.0, 0.0, 0.0)))
+    torch.cuda.synchronize()
+    assert torch.isfinite(f.shaded).all()

@@ -80,7 +80,7 @@ def c0():
     return grid, S.palette()
 
 
-@pytest.mark.parametrize("traversal", [vt.VT_TRAVERSAL_FLAT, vt.VT_TRAVERSAL_BITMASK])
+@pytest.mark.parametrize("traversal", [vt.VT_TRAVERSAL_FLAT, vt.VT_TRAVERSAL_BITMASK, vt.VT_TRAVERSAL_BITMASK8])
 @pytest.mark.parametrize("flags", [0, vt.VT_FLAG_GBUF_UNORM8])
 def test_c0_full_frame(ctx, c0, traversal, flags):
     grid, pal = c0

@@ -223,6 +223,58 @@ def test_exact_tie_skips_the_corner_blocks():
     assert list(h.hit_pos) == [8.0, 8.0, 5.0]
 
 
+ANCHOR = f32(f32(8.0) * f32(0.999))  # within after a negative crossing (map.glsl:162)
+
+
+def test_exact_two_axis_tie_on_negative_axes():
+    # d = (-1, -1, -0.5) from (3.75,3.75,3.75) in a 4^3 world: o*8 = 30, so
+    # iteration 0 looks up block (3,3,3) and switches with within = (6,6,6),
+    # G = (24,24,24).  t = (0 - 6) * (1/d) = (6, 6, 12): a tie of x and y;
+    # minIdx = y (t.x < t.y is false), within += d*6 = (0, 0, 3), then
+    # within.y = 8*0.999.  Iteration 1 looks up pos = (24, 16+7, 24+3) ->
+    # block (3,2,3): unlike a positive tie, one corner block IS looked up,
+    # and the x crossing follows as its own step with t.x = (0-0)*(-1) = -0
+    # (the smallest t), moving nothing but x.  Iteration 2: block (2,2,3).
+    # Then t = (7.992, 7.992, 6): z; within = (1.992, 1.992, 0) -> anchor
+    # on z; block (2,2,2); t = (1.992, 1.992, 15.98): y (x/y tie again),
+    # block (2,1,2); t.x = -0: x, block (1,1,2).
+    h, path = O.trace(world(4), (3.75, 3.75, 3.75), (-1, -1, -0.5), 100, 16)
+    blocks = [tuple(p >> 3) for p in path]
+    assert blocks[:6] == [(3, 3, 3), (3, 2, 3), (2, 2, 3), (2, 2, 2), (2, 1, 2), (1, 1, 2)]
+    assert (2, 3, 3) not in blocks
+    # the corner block (3,2,3) is hit with the y face (normal +y), hit_pos =
+    # G + within = (24 + 0, 16 + 8*0.999, 24 + 3)
+    h, _ = O.trace(world(4, [(3, 2, 3)]), (3.75, 3.75, 3.75), (-1, -1, -0.5), 100)
+    assert h.steps == 2 and h.face == 4 and list(h.block) == [3, 2, 3]
+    assert list(h.hit_pos) == [24.0, f32(f32(16.0) + ANCHOR), 27.0]
+    assert h.tie_margin == 0.0
+    # block (2,2,3) after the t = -0 step: x face (normal +x), within.x
+    # re-anchored, y still the anchor
+    h, _ = O.trace(world(4, [(2, 2, 3)]), (3.75, 3.75, 3.75), (-1, -1, -0.5), 100)
+    assert h.steps == 3 and h.face == 2 and list(h.block) == [2, 2, 3]
+    assert list(h.hit_pos) == [f32(f32(16.0) + ANCHOR), f32(f32(16.0) + ANCHOR), 27.0]
+
+
+def test_exact_three_axis_tie_on_negative_axes():
+    # d = (-1,-1,-1) from (3.75,3.75,3.75): t = (6,6,6), minIdx = z (both
+    # strict comparisons fail); within = (0,0,0) then z anchored.  The next
+    # two steps have t = -0 on x and y: y first (t.x < t.y is false for
+    # -0 < -0, t.y < t.z holds), then x.  Every diagonal move therefore
+    # looks up the z-neighbour and the yz-neighbour before the diagonal
+    # block: z, y, x order, three lookups per diagonal step.
+    h, path = O.trace(world(4), (3.75, 3.75, 3.75), (-1, -1, -1), 100, 16)
+    blocks = [tuple(p >> 3) for p in path]
+    assert blocks == [(3, 3, 3), (3, 3, 2), (3, 2, 2), (2, 2, 2), (2, 2, 1), (2, 1, 1),
+                      (1, 1, 1), (1, 1, 0), (1, 0, 0), (0, 0, 0)]
+    assert h.data == 0 and h.steps == 10
+    h, _ = O.trace(world(4, [(3, 2, 2)]), (3.75, 3.75, 3.75), (-1, -1, -1), 100)
+    assert h.steps == 3 and h.face == 4 and list(h.block) == [3, 2, 2]
+    a = f32(f32(16.0) + ANCHOR)
+    assert list(h.hit_pos) == [24.0, a, a]
+    h, _ = O.trace(world(4, [(2, 2, 2)]), (3.75, 3.75, 3.75), (-1, -1, -1), 100)
+    assert h.steps == 4 and h.face == 2 and list(h.hit_pos) == [a, a, a]
+
+
 def test_outside_start_is_a_miss_without_lookups():
     h, _ = O.trace(world(4), (-1.0, 0.5, 0.5), (1, 0, 0), 100)
     assert h.steps == 0 and h.data == 0

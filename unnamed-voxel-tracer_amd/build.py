@@ -55,10 +55,27 @@ def _run(cmd, verbose):
 
 
 def build(verbose: bool = True) -> None:
+    """Compiles every HIP source to an object in parallel (one hipcc each),
+    then links libvt.so; each translation unit holds its own kernels."""
+    from concurrent.futures import ThreadPoolExecutor
+
     os.makedirs(LIB, exist_ok=True)
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
-    _run([hipcc, *HIP_FLAGS, "-o", os.path.join(LIB, "libvt.so"), *srcs], verbose)
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+    objs = []
+    cmds = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        objs.append(obj)
+        cmds.append([hipcc, *compile_flags, "-c", "-o", obj, os.path.join(CSRC, src)])
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0")) or os.cpu_count() or 1, 16))
+    with ThreadPoolExecutor(jobs) as ex:
+        for f in [ex.submit(_run, c, verbose) for c in cmds]:
+            f.result()
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o",
+          os.path.join(LIB, "libvt.so"), *objs], verbose)
     _run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-ffp-contract=off", "-Wall",
           "-o", os.path.join(LIB, "libvt_scenes.so"),
           os.path.join(PKG, "scenes", "vt_scenes.c"), "-lm", "-lpthread"], verbose)

@@ -49,6 +49,7 @@ void free_world(vt_world* w) {
   dev_free(w->vox);
   dev_free(w->l0);
   dev_free(w->l1);
+  dev_free(w->l1b);
   dev_free(w->palette);
   dev_free(w->chunks);
   dev_free(w->pool);
@@ -89,6 +90,7 @@ DenseWorld dense_view(const vt_world* w) {
   d.vox = w->vox;
   d.l0 = w->l0;
   d.l1 = w->l1;
+  d.l1b = w->l1b;
   d.palette = w->palette;
   d.dim = w->dim;
   uint32_t bits = 0;
@@ -150,7 +152,7 @@ int check_params(const vt_render_params& p, const vt_world* world, const char* w
   if ((p.grad_albedo && !aligned(p.grad_albedo, 4)) || (p.grad_sun && !aligned(p.grad_sun, 4)) ||
       (p.loss && !aligned(p.loss, 4)))
     return fail(VT_ERR_MISALIGNED, "%s: gradient outputs", who);
-  if (p.traversal > VT_TRAVERSAL_BITMASK)
+  if (p.traversal > VT_TRAVERSAL_BITMASK8)
     return fail(VT_ERR_INVALID_ARGUMENT, "%s: unknown traversal %u", who, p.traversal);
   if (world->kind == WorldKind::Brickmap && p.traversal != VT_TRAVERSAL_FLAT)
     return fail(VT_ERR_UNSUPPORTED, "%s: brickmap worlds trace with VT_TRAVERSAL_FLAT", who);
@@ -159,7 +161,19 @@ int check_params(const vt_render_params& p, const vt_world* world, const char* w
   if (p.albedo && !aligned(p.albedo, 4)) return fail(VT_ERR_MISALIGNED, "%s: albedo", who);
   for (int i = 0; i < 3; i++)
     if (!std::isfinite(p.sun[i])) return fail(VT_ERR_INVALID_ARGUMENT, "%s: sun not finite", who);
+  if ((p.flags & VT_FLAG_LAMBERT) && !sun_normalizable(p.sun))
+    return fail(VT_ERR_INVALID_ARGUMENT,
+                "%s: VT_FLAG_LAMBERT normalises the sun: its squared length must be a nonzero "
+                "finite binary32 value", who);
   return VT_OK;
+}
+
+bool sun_normalizable(const float sun[3]) {
+  // the kernels' len3: x*x + y*y + z*z rounded per operation
+  float s = sun[0] * sun[0];
+  s = s + sun[1] * sun[1];
+  s = s + sun[2] * sun[2];
+  return std::isfinite(s) && s > 0.0f;
 }
 
 int check_frame(const vt_frame* frame, uint32_t flags, const char* who) {
@@ -259,12 +273,22 @@ int vt_create(vt_ctx** out, int device) {
   c->device = device;
   c->compute_units = prop.multiProcessorCount;
   std::snprintf(c->name, sizeof c->name, "%s", prop.gcnArchName);
+  c->stage = nullptr;
+  e = hipMalloc(&c->stage, kCtxStageBytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    delete c;
+    return fail(VT_ERR_OUT_OF_MEMORY, "vt_create: %zu bytes of staging: %s", kCtxStageBytes,
+                hipGetErrorString(e));
+  }
   *out = c;
   return VT_OK;
 }
 
 int vt_destroy(vt_ctx* ctx) {
   if (!ctx) return fail(VT_ERR_NULL_POINTER, "vt_destroy: ctx is NULL");
+  (void)hipSetDevice(ctx->device);
+  dev_free(ctx->stage);
   delete ctx;
   return VT_OK;
 }
@@ -306,6 +330,7 @@ int vt_world_create_dense_u8(vt_ctx* ctx, const uint8_t* grid, uint32_t dim,
   if ((st = dev_alloc(w, (void**)&w->vox, n, "voxels")) != VT_OK ||
       (st = dev_alloc(w, (void**)&w->l0, n / 8, "L0 masks")) != VT_OK ||
       (st = dev_alloc(w, (void**)&w->l1, n / 512, "L1 masks")) != VT_OK ||
+      (st = dev_alloc(w, (void**)&w->l1b, vt::l1b_words(dim) * 8, "8^3-brick masks")) != VT_OK ||
       (st = dev_alloc(w, (void**)&w->palette, 256 * 4, "palette")) != VT_OK) {
     free_world(w);
     return st;
@@ -319,7 +344,7 @@ int vt_world_create_dense_u8(vt_ctx* ctx, const uint8_t* grid, uint32_t dim,
   }
   e = hipMemcpy(lin, grid, n, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(w->palette, palette_rgba8, 256 * 4, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = vt::launch_build_dense(lin, dim, w->vox, w->l0, w->l1, nullptr);
+  if (e == hipSuccess) e = vt::launch_build_dense(lin, dim, w->vox, w->l0, w->l1, w->l1b, nullptr);
   if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
   dev_free(lin);
   if (e != hipSuccess) {

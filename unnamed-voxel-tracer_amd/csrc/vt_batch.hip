@@ -106,32 +106,91 @@ namespace {
 using vt::fail;
 using vt::hip_fail;
 
-hipError_t enqueue(const vt_batch* b, hipStream_t s) {
-  const vt::DenseWorld w = vt::dense_view(b->world);
-  hipLaunchKernelGGL(vt::k_batch_reset, dim3(1), dim3(64), 0, s, b->d_queues);
+// Queue reset + persistent launch over a device view table.
+hipError_t launch_batch(const vt_world* world, const vt::ShadeConsts& sc,
+                        const vt::BatchView* d_views, uint32_t n_views, uint32_t total_tiles,
+                        uint32_t* d_queues, const vt_tuning& tuning, uint32_t nblocks,
+                        unsigned long long* stats, hipStream_t s) {
+  const vt::DenseWorld w = vt::dense_view(world);
+  hipLaunchKernelGGL(vt::k_batch_reset, dim3(1), dim3(64), 0, s, d_queues);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const dim3 grid(b->nblocks), block(256);
-  const uint32_t tile = b->tuning.tile, chunk = b->tuning.chunk;
-#define VT_BATCH_LAUNCH(L, S, M)                                                         \
-  hipLaunchKernelGGL((vt::k_render_batch<vt::L, S, vt::M>), grid, block, 0, s, w, b->sc,     \
-                     b->d_views, b->n_views, b->total_tiles, b->d_queues, chunk, tile, b->stats)
+  const dim3 grid(nblocks), block(256);
+  const uint32_t tile = tuning.tile, chunk = tuning.chunk;
+#define VT_BATCH_LAUNCH(L, S, M)                                                           \
+  hipLaunchKernelGGL((vt::k_render_batch<vt::L, S, vt::M>), grid, block, 0, s, w, sc, d_views, \
+                     n_views, total_tiles, d_queues, chunk, tile, stats)
 #define VT_BATCH_SHADE(L, S)                                                            \
-  switch (vt::shade_mode(b->sc.flags)) {                                                \
+  switch (vt::shade_mode(sc.flags)) {                                                   \
     case vt::kShadeLambertGrad: VT_BATCH_LAUNCH(L, S, kShadeLambertGrad); break;        \
     case vt::kShadeLambert: VT_BATCH_LAUNCH(L, S, kShadeLambert); break;                \
     default: VT_BATCH_LAUNCH(L, S, kShadeRef);                                          \
   }
-  if (b->tuning.traversal == VT_TRAVERSAL_BITMASK) {
-    if (b->stats) VT_BATCH_SHADE(BitmaskLookup, true)
+  if (tuning.traversal == VT_TRAVERSAL_BITMASK) {
+    if (stats) VT_BATCH_SHADE(BitmaskLookup, true)
     else VT_BATCH_SHADE(BitmaskLookup, false)
+  } else if (tuning.traversal == VT_TRAVERSAL_BITMASK8) {
+    if (stats) VT_BATCH_SHADE(BitmaskLookup8, true)
+    else VT_BATCH_SHADE(BitmaskLookup8, false)
   } else {
-    if (b->stats) VT_BATCH_SHADE(FlatLookup, true)
+    if (stats) VT_BATCH_SHADE(FlatLookup, true)
     else VT_BATCH_SHADE(FlatLookup, false)
   }
 #undef VT_BATCH_SHADE
 #undef VT_BATCH_LAUNCH
   return hipGetLastError();
+}
+
+hipError_t enqueue(const vt_batch* b, hipStream_t s) {
+  return launch_batch(b->world, b->sc, b->d_views, b->n_views, b->total_tiles, b->d_queues,
+                      b->tuning, b->nblocks, b->stats, s);
+}
+
+// Validation shared by vt_batch_create and vt_batch_render_noalloc, and the
+// view record of view i: *tiles_out receives the total tile count.
+int check_batch(vt_ctx* ctx, const vt_world* world, const vt_view* views, uint32_t n_views,
+                const vt_render_params* params, const vt_tuning* tuning,
+                unsigned long long* stats, uint32_t max_views, const char* who,
+                vt_render_params* p_out) {
+  if (!ctx || !world || !views || !params || !tuning)
+    return fail(VT_ERR_NULL_POINTER, "%s: ctx, world, views, params, tuning required", who);
+  if (world->ctx != ctx) return fail(VT_ERR_WRONG_CONTEXT, "%s: world from another ctx", who);
+  if (world->kind != WorldKind::Dense)
+    return fail(VT_ERR_UNSUPPORTED, "%s: batched rendering needs a dense world", who);
+  if (n_views == 0 || n_views > max_views)
+    return fail(VT_ERR_INVALID_ARGUMENT, "%s: n_views %u outside [1, %u]", who, n_views,
+                max_views);
+  const vt_tuning t = *tuning;
+  if (t.tile < VT_TILE_8X8 || t.tile > VT_TILE_32X2 || t.chunk < 1 || t.chunk > 64 ||
+      t.traversal > VT_TRAVERSAL_BITMASK8 || t.blocks_per_cu < 1 || t.blocks_per_cu > 16)
+    return fail(VT_ERR_INVALID_ARGUMENT, "%s: tuning out of range", who);
+  if (stats && !vt::aligned(stats, 8)) return fail(VT_ERR_MISALIGNED, "%s: stats", who);
+  vt_render_params p = *params;
+  p.traversal = t.traversal;
+  int st = vt::check_params(p, world, who);
+  if (st != VT_OK) return st;
+  for (uint32_t i = 0; i < n_views; i++)
+    if ((st = vt::check_image_size(views[i].width, views[i].height, who)) != VT_OK ||
+        (st = vt::check_frame(&views[i].frame, p.flags, who)) != VT_OK)
+      return st;
+  *p_out = p;
+  return VT_OK;
+}
+
+// View record of views[i] whose tiles start at *tiles (advanced past them).
+int view_record(const vt_view& v, uint32_t log2_tw, uint64_t* tiles, vt::BatchView* b,
+                const char* who) {
+  const uint32_t tw = 1u << log2_tw, th = 64u >> log2_tw;
+  std::memset(b, 0, sizeof *b);
+  b->v = vt::view_consts(&v.camera, v.width, v.height);
+  b->out = vt::frame_out(&v.frame);
+  b->out.stats = nullptr;
+  b->tiles_x = (v.width + tw - 1) / tw;
+  b->tile_base = (uint32_t)*tiles;
+  *tiles += (uint64_t)b->tiles_x * ((v.height + th - 1) / th);
+  if (*tiles >= (1ull << 31))
+    return fail(VT_ERR_INVALID_ARGUMENT, "%s: more than 2^31 tiles", who);
+  return VT_OK;
 }
 
 void release(vt_batch* b) {
@@ -153,41 +212,16 @@ int vt_batch_create(vt_ctx* ctx, const vt_world* world, const vt_view* views, ui
                     unsigned long long* stats, vt_batch** out) {
   if (!out) return fail(VT_ERR_NULL_POINTER, "vt_batch_create: out is NULL");
   *out = nullptr;
-  if (!ctx || !world || !views || !params || !tuning)
-    return fail(VT_ERR_NULL_POINTER, "vt_batch_create: ctx, world, views, params, tuning required");
-  if (world->ctx != ctx) return fail(VT_ERR_WRONG_CONTEXT, "vt_batch_create: world from another ctx");
-  if (world->kind != WorldKind::Dense)
-    return fail(VT_ERR_UNSUPPORTED, "vt_batch_create: batched rendering needs a dense world");
-  if (n_views == 0 || n_views > 65536)
-    return fail(VT_ERR_INVALID_ARGUMENT, "vt_batch_create: n_views %u outside [1, 65536]", n_views);
-  const vt_tuning t = *tuning;
-  if (t.tile < VT_TILE_8X8 || t.tile > VT_TILE_32X2 || t.chunk < 1 || t.chunk > 64 ||
-      t.traversal > VT_TRAVERSAL_BITMASK || t.blocks_per_cu < 1 || t.blocks_per_cu > 16)
-    return fail(VT_ERR_INVALID_ARGUMENT, "vt_batch_create: tuning out of range");
-  if (stats && !vt::aligned(stats, 8)) return fail(VT_ERR_MISALIGNED, "vt_batch_create: stats");
-  vt_render_params p = *params;
-  p.traversal = t.traversal;
-  int st = vt::check_params(p, world, "vt_batch_create");
+  vt_render_params p;
+  int st = check_batch(ctx, world, views, n_views, params, tuning, stats, 65536,
+                       "vt_batch_create", &p);
   if (st != VT_OK) return st;
-  const uint32_t tw = 1u << t.tile, th = 64u >> t.tile;
+  const vt_tuning t = *tuning;
   std::vector<vt::BatchView> table(n_views);
   uint64_t tiles = 0;
-  for (uint32_t i = 0; i < n_views; i++) {
-    const vt_view& v = views[i];
-    if ((st = vt::check_image_size(v.width, v.height, "vt_batch_create")) != VT_OK ||
-        (st = vt::check_frame(&v.frame, p.flags, "vt_batch_create")) != VT_OK)
+  for (uint32_t i = 0; i < n_views; i++)
+    if ((st = view_record(views[i], t.tile, &tiles, &table[i], "vt_batch_create")) != VT_OK)
       return st;
-    vt::BatchView& b = table[i];
-    std::memset(&b, 0, sizeof b);
-    b.v = vt::view_consts(&v.camera, v.width, v.height);
-    b.out = vt::frame_out(&v.frame);
-    b.out.stats = nullptr;
-    b.tiles_x = (v.width + tw - 1) / tw;
-    b.tile_base = (uint32_t)tiles;
-    tiles += (uint64_t)b.tiles_x * ((v.height + th - 1) / th);
-    if (tiles >= (1ull << 31))
-      return fail(VT_ERR_INVALID_ARGUMENT, "vt_batch_create: more than 2^31 tiles");
-  }
   VT_HIP(hipSetDevice(ctx->device));
   vt_batch* b = new (std::nothrow) vt_batch;
   if (!b) return fail(VT_ERR_OUT_OF_MEMORY, "vt_batch_create: host allocation");
@@ -269,6 +303,10 @@ int vt_batch_autotune(vt_ctx* ctx, const vt_world* world, const vt_view* views,
   if (!ctx) return fail(VT_ERR_NULL_POINTER, "vt_batch_autotune: ctx is NULL");
   if (reps < 1 || reps > 1000)
     return fail(VT_ERR_INVALID_ARGUMENT, "vt_batch_autotune: reps %d outside [1, 1000]", reps);
+  if (params && (params->flags & VT_FLAG_GRAD_L2))
+    return fail(VT_ERR_INVALID_ARGUMENT,
+                "vt_batch_autotune: VT_FLAG_GRAD_L2 accumulates into the caller's gradients on "
+                "every timed launch; tune with VT_FLAG_LAMBERT alone (same kernel shape)");
   VT_HIP(hipSetDevice(ctx->device));
   hipStream_t s;
   VT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -280,7 +318,7 @@ int vt_batch_autotune(vt_ctx* ctx, const vt_world* world, const vt_view* views,
   int st = VT_OK;
   const uint32_t tiles[] = {VT_TILE_8X8, VT_TILE_16X4, VT_TILE_32X2};
   const uint32_t chunks[] = {1, 2, 4, 8};
-  const uint32_t travs[] = {VT_TRAVERSAL_FLAT, VT_TRAVERSAL_BITMASK};
+  const uint32_t travs[] = {VT_TRAVERSAL_FLAT, VT_TRAVERSAL_BITMASK, VT_TRAVERSAL_BITMASK8};
   for (uint32_t tr : travs)
     for (uint32_t ti : tiles)
       for (uint32_t ch : chunks) {

@@ -19,6 +19,7 @@
 //             then the compact albedo index, when built, is recomputed in
 //             place (its numbering changes with the occupancy).
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <unordered_map>
@@ -33,6 +34,13 @@ __global__ __launch_bounds__(256) void k_scatter_u32(uint32_t* __restrict__ dst,
                                                      const uint32_t* __restrict__ val, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[addr[i]] = val[i];
+}
+
+// dst[addr[i]] = 0 for i < n (undoing k_scatter_u32 of new chunk entries).
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ dst,
+                                                  const uint64_t* __restrict__ addr, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[addr[i]] = 0u;
 }
 
 __global__ __launch_bounds__(256) void k_scatter_u8(uint8_t* __restrict__ dst,
@@ -166,6 +174,14 @@ int grow_pool(vt_world* w, uint64_t blocks, hipStream_t s, const char* who) {
   if (blocks <= cap) return VT_OK;
   uint64_t ncap = cap ? cap : 1;
   while (ncap < blocks) ncap *= 2;
+  // An optional budget for the block pool (bytes): a growth past it fails
+  // like a failed allocation, before anything changes.
+  if (const char* lim = std::getenv("VT_POOL_LIMIT_BYTES")) {
+    const unsigned long long limit = std::strtoull(lim, nullptr, 10);
+    if (limit && ncap * 512u * 4u > limit)
+      return fail(VT_ERR_OUT_OF_MEMORY, "%s: the pool would grow to %llu bytes, past "
+                  "VT_POOL_LIMIT_BYTES=%llu", who, (unsigned long long)(ncap * 512u * 4u), limit);
+  }
   uint32_t* np = nullptr;
   hipError_t e = hipMalloc((void**)&np, ncap * 512u * 4u);
   if (e != hipSuccess) {
@@ -242,16 +258,26 @@ int set_brickmap(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint3
   std::unordered_map<uint64_t, uint32_t> last;  // pool word -> position in `writes`
   std::vector<uint64_t> addr, oaddr;            // pool word, occupancy byte (Morton)
   std::vector<uint32_t> val;
+  // chunks this call allocates: kept here and entered into the host chunk
+  // table only once the device holds them, so a failed call leaves the
+  // world as it was
+  std::unordered_map<uint64_t, uint32_t> fresh;
   uint64_t used = w->pool_words / 512u;
   last.reserve(n);
   for (uint32_t i = 0; i < n; i++) {
     const uint32_t x = xyz[3 * (size_t)i], y = xyz[3 * (size_t)i + 1], z = xyz[3 * (size_t)i + 2];
     const uint64_t ci = (uint64_t)(x / 8u) + M * ((uint64_t)(y / 8u) + (uint64_t)M * (z / 8u));
-    uint32_t& entry = w->chunk_mirror[ci];
-    if (entry == 0) {  // get_block_for_chunk: allocate on first touch
-      if (used >= 0xFFFFFFFEull) return fail(VT_ERR_OUT_OF_MEMORY, "vt_world_set_voxels: pool");
-      entry = (uint32_t)(++used);
-      new_chunks.emplace_back(ci, entry);
+    uint32_t entry = w->chunk_mirror[ci];
+    if (entry == 0) {
+      auto f = fresh.find(ci);
+      if (f != fresh.end()) {
+        entry = f->second;
+      } else {  // get_block_for_chunk: allocate on first touch
+        if (used >= 0xFFFFFFFEull) return fail(VT_ERR_OUT_OF_MEMORY, "vt_world_set_voxels: pool");
+        entry = (uint32_t)(++used);
+        fresh.emplace(ci, entry);
+        new_chunks.emplace_back(ci, entry);
+      }
     }
     const uint64_t a = (uint64_t)(entry - 1u) * 512u + (x % 8u) + 8u * ((y % 8u) + 8u * (z % 8u));
     auto it = last.find(a);
@@ -265,8 +291,7 @@ int set_brickmap(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint3
     }
   }
   int st = vt::grow_pool(w, used, s, "vt_world_set_voxels: pool growth");
-  if (st != VT_OK) return st;  // (the mirror keeps the allocations: same as a
-                               //  reference resize panic, the world is unusable)
+  if (st != VT_OK) return st;  // nothing written: the world is unchanged
   std::vector<uint64_t> caddr;
   std::vector<uint32_t> cval;
   for (auto& c : new_chunks) {
@@ -290,8 +315,23 @@ int set_brickmap(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint3
   if (!caddr.empty())
     hipLaunchKernelGGL(vt::k_scatter_u32, dim3(blocks_for(caddr.size(), 256)), dim3(256), 0, s,
                        w->chunks, d_caddr, d_cval, (uint32_t)caddr.size());
-  VT_HIP(hipGetLastError());
-  VT_HIP(hipStreamSynchronize(s));
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    // Undo what may have reached the device: new chunk entries back to 0
+    // and the new blocks zero again (edits allocate zero blocks), so the
+    // device agrees with the unchanged host chunk table and pool count.
+    if (!caddr.empty())
+      hipLaunchKernelGGL(vt::k_fill_u32, dim3(blocks_for(caddr.size(), 256)), dim3(256), 0, s,
+                         w->chunks, d_caddr, (uint32_t)caddr.size());
+    const uint64_t b0 = w->pool_words / 512u;
+    if (used > b0)
+      (void)hipMemsetAsync(w->pool + b0 * 512u, 0, (used - b0) * 512u * 4u, s);
+    (void)hipStreamSynchronize(s);
+    (void)hipGetLastError();
+    return hip_fail(e, "vt_world_set_voxels: scatter");
+  }
+  for (auto& c : new_chunks) w->chunk_mirror[c.first] = c.second;
   w->pool_words = used * 512u;
   return VT_OK;
 }
@@ -304,7 +344,7 @@ int set_dense(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint32_t
                   values[i], i);
   std::unordered_map<uint64_t, uint32_t> last;
   std::vector<uint64_t> addr;
-  std::vector<uint32_t> val, bricks, regions;
+  std::vector<uint32_t> val, bricks, regions, regions8;
   last.reserve(n);
   for (uint32_t i = 0; i < n; i++) {
     const uint32_t code =
@@ -316,6 +356,7 @@ int set_dense(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint32_t
       val.push_back(values[i]);
       bricks.push_back(code >> 6);
       regions.push_back(code >> 12);
+      regions8.push_back(code >> 15);
     } else {
       val[it->second] = values[i];
     }
@@ -324,13 +365,16 @@ int set_dense(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint32_t
   bricks.erase(std::unique(bricks.begin(), bricks.end()), bricks.end());
   std::sort(regions.begin(), regions.end());
   regions.erase(std::unique(regions.begin(), regions.end()), regions.end());
+  std::sort(regions8.begin(), regions8.end());
+  regions8.erase(std::unique(regions8.begin(), regions8.end()), regions8.end());
   Staging stg;
   uint64_t* d_addr;
-  uint32_t *d_val, *d_bricks, *d_regions;
+  uint32_t *d_val, *d_bricks, *d_regions, *d_regions8;
   VT_HIP(stg.upload(addr, &d_addr, s));
   VT_HIP(stg.upload(val, &d_val, s));
   VT_HIP(stg.upload(bricks, &d_bricks, s));
   VT_HIP(stg.upload(regions, &d_regions, s));
+  VT_HIP(stg.upload(regions8, &d_regions8, s));
   if (!addr.empty()) {
     hipLaunchKernelGGL(vt::k_scatter_u8, dim3(blocks_for(addr.size(), 256)), dim3(256), 0, s,
                        w->vox, d_addr, d_val, (uint32_t)addr.size());
@@ -338,6 +382,8 @@ int set_dense(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint32_t
                        dim3(256), 0, s, w->vox, w->l0, d_bricks, (uint32_t)bricks.size());
     hipLaunchKernelGGL((vt::k_rebuild_mask<uint64_t>), dim3(blocks_for(regions.size(), 4)),
                        dim3(256), 0, s, w->l0, w->l1, d_regions, (uint32_t)regions.size());
+    VT_HIP(vt::launch_build_l1b(w->l0, (uint64_t)w->dim * w->dim * w->dim / 64u, d_regions8,
+                                (uint32_t)regions8.size(), w->l1b, s));
   }
   VT_HIP(hipGetLastError());
   if (w->prefix && !addr.empty()) {  // keep the compact index current
@@ -382,21 +428,22 @@ int vt_world_get_voxels(vt_ctx* ctx, const vt_world* world, const uint32_t* xyz,
   if (st != VT_OK || n == 0) return st;
   VT_HIP(hipSetDevice(ctx->device));
   const hipStream_t s = (hipStream_t)stream;
-  std::vector<uint32_t> coords(xyz, xyz + 3 * (size_t)n);
-  Staging stg;
-  uint32_t *d_xyz, *d_out;
-  VT_HIP(stg.upload(coords, &d_xyz, s));
-  VT_HIP(hipMalloc((void**)&d_out, (size_t)n * 4));
-  stg.bufs.push_back(d_out);
-  if (world->kind == WorldKind::Dense)
-    hipLaunchKernelGGL(vt::k_gather_dense, dim3(blocks_for(n, 256)), dim3(256), 0, s,
-                       vt::dense_view(world), d_xyz, n, d_out);
-  else
-    hipLaunchKernelGGL(vt::k_gather_brickmap, dim3(blocks_for(n, 256)), dim3(256), 0, s,
-                       vt::brickmap_view(world), d_xyz, n, d_out);
-  VT_HIP(hipGetLastError());
-  VT_HIP(hipMemcpyAsync(values, d_out, (size_t)n * 4, hipMemcpyDeviceToHost, s));
-  VT_HIP(hipStreamSynchronize(s));
+  // the context's staging: coordinates, then values, kCtxStageVoxels at a time
+  uint32_t* d_xyz = (uint32_t*)ctx->stage;
+  uint32_t* d_out = d_xyz + 3u * (size_t)kCtxStageVoxels;
+  for (uint32_t i0 = 0; i0 < n; i0 += kCtxStageVoxels) {
+    const uint32_t m = std::min(n - i0, kCtxStageVoxels);
+    VT_HIP(hipMemcpyAsync(d_xyz, xyz + 3 * (size_t)i0, 12 * (size_t)m, hipMemcpyHostToDevice, s));
+    if (world->kind == WorldKind::Dense)
+      hipLaunchKernelGGL(vt::k_gather_dense, dim3(blocks_for(m, 256)), dim3(256), 0, s,
+                         vt::dense_view(world), d_xyz, m, d_out);
+    else
+      hipLaunchKernelGGL(vt::k_gather_brickmap, dim3(blocks_for(m, 256)), dim3(256), 0, s,
+                         vt::brickmap_view(world), d_xyz, m, d_out);
+    VT_HIP(hipGetLastError());
+    VT_HIP(hipMemcpyAsync(values + i0, d_out, 4 * (size_t)m, hipMemcpyDeviceToHost, s));
+    VT_HIP(hipStreamSynchronize(s));
+  }
   return VT_OK;
 }
 
@@ -434,8 +481,7 @@ int vt_pick(vt_ctx* ctx, const vt_world* world, const vt_camera* camera, int32_t
   VT_HIP(hipSetDevice(ctx->device));
   const hipStream_t s = (hipStream_t)stream;
   const vt::ViewConsts v = vt::view_consts(camera, 1, 1);
-  vt_pick_result* d = nullptr;
-  VT_HIP(hipMalloc((void**)&d, sizeof *d));
+  vt_pick_result* d = (vt_pick_result*)ctx->stage;  // the context's staging
   if (world->kind == WorldKind::Dense)
     hipLaunchKernelGGL(vt::k_pick_dense, dim3(1), dim3(64), 0, s, vt::dense_view(world), v,
                        (int)max_steps, d);
@@ -445,7 +491,6 @@ int vt_pick(vt_ctx* ctx, const vt_world* world, const vt_camera* camera, int32_t
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(result, d, sizeof *d, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  vt::dev_free(d);
   if (e != hipSuccess) return hip_fail(e, "vt_pick");
   return VT_OK;
 }

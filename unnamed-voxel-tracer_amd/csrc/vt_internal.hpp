@@ -9,10 +9,19 @@
 #include "../../include/vt.h"
 #include "vt_trace.hpp"
 
+// Device staging owned by a context, allocated once by vt_create: the pick
+// result and the coordinates / values of vt_world_get_voxels (in chunks of
+// kCtxStageVoxels), so neither allocates per call.  Calls that use it
+// return only after their stream work is done, so calls made one after
+// another on the context's thread never overlap on it.
+constexpr size_t kCtxStageBytes = 256u * 1024u;
+constexpr uint32_t kCtxStageVoxels = (uint32_t)(kCtxStageBytes / 16u);  // 12 B xyz + 4 B value
+
 struct vt_ctx {
   int device;
   int compute_units;
   char name[256];
+  void* stage;  // kCtxStageBytes of device memory, 256-byte aligned
 };
 
 enum class WorldKind { Dense, Brickmap };
@@ -26,6 +35,7 @@ struct vt_world {
   uint8_t* vox;
   uint64_t* l0;
   uint64_t* l1;
+  uint64_t* l1b;  // 8^3-brick level, l1b_words(dim) words
   uint32_t* palette;
   // brickmap
   uint32_t* chunks;
@@ -93,6 +103,9 @@ BrickmapWorld brickmap_view(const vt_world* w);
 // Host-side validation and translation of the C-ABI structs.
 int check_image_size(uint32_t W, uint32_t H, const char* who);
 int check_params(const vt_render_params& p, const vt_world* world, const char* who);
+// The Lambert modes divide by |sun|: false when its binary32 squared length
+// is 0 (or overflows), where that division has no finite result.
+bool sun_normalizable(const float sun[3]);
 int check_frame(const vt_frame* frame, uint32_t flags, const char* who);
 ViewConsts view_consts(const vt_camera* camera, uint32_t W, uint32_t H);
 ShadeConsts shade_consts(const vt_render_params& p);
@@ -115,7 +128,17 @@ hipError_t launch_build_occupancy(const BrickmapWorld& w, uint8_t* occ, hipStrea
 // the contents and zeroing the new blocks; `who` names the caller in errors.
 int grow_pool(vt_world* w, uint64_t blocks, hipStream_t s, const char* who);
 hipError_t launch_build_dense(const uint8_t* linear, uint32_t dim, uint8_t* vox, uint64_t* l0,
-                              uint64_t* l1, hipStream_t stream);
+                              uint64_t* l1, uint64_t* l1b, hipStream_t stream);
+// Words of the 8^3-brick level of a dim^3 world: one per 32^3 region (one
+// partial word for dim 16).
+inline uint64_t l1b_words(uint32_t dim) {
+  const uint64_t n = (uint64_t)dim * dim * dim / 32768u;
+  return n ? n : 1u;
+}
+// Rebuilds the listed L1b words (all l1b_words(dim) when words is null)
+// from the L0 words, n0 of them.
+hipError_t launch_build_l1b(const uint64_t* l0, uint64_t n0, const uint32_t* words,
+                            uint32_t nwords, uint64_t* l1b, hipStream_t stream);
 // Exclusive prefix sum of popcount(l0[i]) into prefix[0..nwords); scratch
 // holds ceil(nwords/1024) + 1 words; *total receives the sum (device).
 hipError_t launch_compact_prefix(const uint64_t* l0, uint64_t nwords, uint32_t* prefix,

@@ -20,12 +20,19 @@ __device__ __forceinline__ bool record_hit(const DenseWorld& w, uint32_t voxel) 
   return voxel < w.dim * w.dim * w.dim;
 }
 
+// Hit record i in one 12-byte access (global_load_dwordx3 needs 4-byte
+// alignment only): a wave reads its 64 records as one contiguous 768-byte run.
+__device__ __forceinline__ uint3 load_record(const uint32_t* __restrict__ hits, uint32_t i) {
+  return reinterpret_cast<const uint3*>(hits)[i];
+}
+
 __global__ __launch_bounds__(256) void k_lambert_shade(DenseWorld w, ShadeConsts sc,
                                                        const uint32_t* __restrict__ hits,
                                                        uint32_t npix, float4* __restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npix) return;
-  const uint32_t voxel = hits[3 * (size_t)i], info = hits[3 * (size_t)i + 2];
+  const uint3 rec = load_record(hits, i);
+  const uint32_t voxel = rec.x, info = rec.z;
   float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (record_hit(w, voxel)) {
     const int face = (int)(info & 7u);
@@ -52,7 +59,8 @@ __global__ __launch_bounds__(256) void k_lambert_backward(DenseWorld w, ShadeCon
   for (uint32_t base = blockIdx.x * 256u; base < npix; base += gridDim.x * 256u) {
     const uint32_t i = base + threadIdx.x;
     if (i < npix) {
-      const uint32_t voxel = hits[3 * (size_t)i], info = hits[3 * (size_t)i + 2];
+      const uint3 rec = load_record(hits, i);
+      const uint32_t voxel = rec.x, info = rec.z;
       const float4 u = up[i];
       if (record_hit(w, voxel)) {
         const int face = (int)(info & 7u);
@@ -123,6 +131,10 @@ int check_lambert_call(vt_ctx* ctx, const vt_world* world, const vt_hit_record* 
     return fail(VT_ERR_UNSUPPORTED, "%s: build the compact index first", who);
   for (int i = 0; i < 3; i++)
     if (!std::isfinite(p->sun[i])) return fail(VT_ERR_INVALID_ARGUMENT, "%s: sun not finite", who);
+  if (!vt::sun_normalizable(p->sun))
+    return fail(VT_ERR_INVALID_ARGUMENT,
+                "%s: Lambert shading normalises the sun: its squared length must be a nonzero "
+                "finite binary32 value", who);
   return VT_OK;
 }
 

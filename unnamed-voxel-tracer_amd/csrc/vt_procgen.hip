@@ -370,31 +370,47 @@ int vt_world_procgen(vt_ctx* ctx, vt_world* world, float offset_x, float offset_
   }
 
   int st = vt::grow_pool(world, walk.used, s, "vt_world_procgen: pool growth");
-  if (st != VT_OK) return st;
+  if (st != VT_OK) return st;  // nothing written: the map is still empty
   // one staging allocation: column states and decorations, then the tree voxels
   uint32_t* d_cols;
   VT_HIP(bufs.alloc((void**)&d_cols, 2 * ncol * 4 + tv.size() * sizeof(vt::TreeVoxel)));
   vt::TreeVoxel* d_tv = (vt::TreeVoxel*)(d_cols + 2 * ncol);
-  VT_HIP(hipMemcpyAsync(world->chunks, walk.mirror.data(), nchunks * 4, hipMemcpyHostToDevice, s));
-  VT_HIP(hipMemcpyAsync(d_cols, walk.cols.data(), 2 * ncol * 4, hipMemcpyHostToDevice, s));
-  if (!tv.empty())
-    VT_HIP(hipMemcpyAsync(d_tv, tv.data(), tv.size() * sizeof(vt::TreeVoxel),
-                          hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(vt::k_procgen_columns, dim3((uint32_t)(M * M)), dim3(64), 0, s, dim,
-                     world->chunks, world->pool, d_vh, d_cols, d_cols + ncol);
-  VT_HIP(hipGetLastError());
-  if (!tv.empty()) {
+  // From here on the device map changes; the host chunk table and pool count
+  // follow only once all of it succeeded.
+  hipError_t e = hipMemcpyAsync(world->chunks, walk.mirror.data(), nchunks * 4,
+                                hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_cols, walk.cols.data(), 2 * ncol * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && !tv.empty())
+    e = hipMemcpyAsync(d_tv, tv.data(), tv.size() * sizeof(vt::TreeVoxel), hipMemcpyHostToDevice,
+                       s);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(vt::k_procgen_columns, dim3((uint32_t)(M * M)), dim3(64), 0, s, dim,
+                       world->chunks, world->pool, d_vh, d_cols, d_cols + ncol);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess && !tv.empty()) {
     hipLaunchKernelGGL(vt::k_procgen_trees, dim3((uint32_t)((tv.size() + 255) / 256)), dim3(256),
                        0, s, world->pool, d_tv, (uint32_t)tv.size());
-    VT_HIP(hipGetLastError());
+    e = hipGetLastError();
+  }
+  // the empty map's occupancy bytes were zero; only the written layers change
+  if (e == hipSuccess && world->occ)
+    e = vt::launch_build_occupancy(vt::brickmap_view(world), world->occ, s,
+                                   std::min(dim, (walk.y_end + 7u) & ~7u));
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    // back to the empty map the call was given: chunk table, written pool
+    // blocks and occupancy bytes zero
+    (void)hipMemsetAsync(world->chunks, 0, nchunks * 4, s);
+    (void)hipMemsetAsync(world->pool, 0, walk.used * 512u * 4u, s);
+    if (world->occ) (void)hipMemsetAsync(world->occ, 0, (size_t)dim * dim * dim, s);
+    (void)hipStreamSynchronize(s);
+    (void)hipGetLastError();
+    return vt::hip_fail(e, "vt_world_procgen: device writes");
   }
   std::memcpy(world->chunk_mirror, walk.mirror.data(), nchunks * 4);
   world->pool_words = walk.used * 512u;
-  // the empty map's occupancy bytes were zero; only the written layers change
-  if (world->occ)
-    VT_HIP(vt::launch_build_occupancy(vt::brickmap_view(world), world->occ, s,
-                                      std::min(dim, (walk.y_end + 7u) & ~7u)));
-  VT_HIP(hipStreamSynchronize(s));
   return VT_OK;
 }
 

@@ -330,6 +330,11 @@ hipError_t launch_render_dense(const DenseWorld& w, uint32_t traversal, const Vi
       launch_dense_shade<BitmaskLookup, true>(w, v, sc, out, tile.log2_w, blocks_x, grid, pack, stream);
     else
       launch_dense_shade<BitmaskLookup, false>(w, v, sc, out, tile.log2_w, blocks_x, grid, pack, stream);
+  } else if (traversal == VT_TRAVERSAL_BITMASK8) {
+    if (stats)
+      launch_dense_shade<BitmaskLookup8, true>(w, v, sc, out, tile.log2_w, blocks_x, grid, pack, stream);
+    else
+      launch_dense_shade<BitmaskLookup8, false>(w, v, sc, out, tile.log2_w, blocks_x, grid, pack, stream);
   } else {
     if (stats)
       launch_dense_shade<FlatLookup, true>(w, v, sc, out, tile.log2_w, blocks_x, grid, pack, stream);

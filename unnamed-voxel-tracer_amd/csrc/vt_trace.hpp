@@ -198,7 +198,8 @@ __device__ __forceinline__ void sky(const float sun[3], float rx, float ry, floa
 struct DenseWorld {
   const uint8_t* vox;       // material byte of voxel (x,y,z) at morton(x,y,z)
   const uint64_t* l0;       // 4^3 bricks: bit per voxel
-  const uint64_t* l1;       // 16^3 regions: bit per nonempty brick
+  const uint64_t* l1;       // 16^3 regions: bit per nonempty 4^3 brick
+  const uint64_t* l1b;      // 32^3 regions: bit per nonempty 8^3 brick
   const uint32_t* palette;  // RGBA8 colour of material k (the solid atlas cube)
   uint32_t dim;             // voxels per axis (MAP_DIMENSION), a power of two
   uint32_t log2dim;
@@ -248,45 +249,65 @@ struct FlatLookup {
   }
 };
 
-struct BitmaskLookup {
-  // keeps the last brick word in registers; a probe in another brick fetches
-  // the region word when the 16^3 region changed, then the brick word only
-  // when the region word says the brick is not empty.
+// The 2-level occupancy probe: a region word (one bit per brick) and the
+// probed voxel's 64-bit leaf word, both kept in registers between probes.
+//  kBrick = 4: 4^3 bricks in 16^3 regions -- region word l1[code >> 12],
+//              bit (code >> 6) & 63; the brick is the leaf word l0[code >> 6].
+//  kBrick = 8: 8^3 bricks in 32^3 regions -- region word l1b[code >> 15],
+//              bit (code >> 9) & 63; an occupied 8^3 brick's voxels are the
+//              8 leaf words l0[code >> 6], of which the one holding the
+//              probed 4^3 sub-brick is fetched.
+// A leaf word is fetched only when its brick is marked occupied, a region
+// word only when the region changes.
+template <uint32_t kBrick>
+struct BitmaskLookupT {
+  static_assert(kBrick == 4 || kBrick == 8, "4^3 or 8^3 bricks");
   static constexpr bool kPrefetch = false;
-  uint32_t brick, region;
+  static constexpr uint32_t kBrickShift = kBrick == 8 ? 9u : 6u;  // log2 codes per brick
+  uint32_t brick, region, leaf;
   uint64_t bword, rword;
+  bool bset;  // the current brick is marked occupied
   __device__ __forceinline__ void reset() {
     brick = 0xFFFFFFFFu;
     region = 0xFFFFFFFFu;
+    leaf = 0xFFFFFFFFu;
     bword = 0;
     rword = 0;
+    bset = false;
   }
   template <bool kStats>
   __device__ __forceinline__ uint32_t probe(const DenseWorld& w, uint32_t code, uint32_t& l1f,
                                             uint32_t& l0f) {
-    const uint32_t bk = code >> 6;
-    if (bk != brick) {
-      brick = bk;
-      const uint32_t rk = code >> 12;
-      if (rk != region) {
-        region = rk;
-        rword = w.l1[rk];
-        if (kStats) l1f++;
+    const uint32_t lf = code >> 6;
+    if (lf != leaf) {
+      leaf = lf;
+      const uint32_t bk = code >> kBrickShift;
+      if (bk != brick) {
+        brick = bk;
+        const uint32_t rk = bk >> 6;
+        if (rk != region) {
+          region = rk;
+          rword = kBrick == 8 ? w.l1b[rk] : w.l1[rk];
+          if (kStats) l1f++;
+        }
+        bset = ((rword >> (bk & 63u)) & 1ull) != 0ull;
       }
       bword = 0;
-      if ((rword >> (bk & 63u)) & 1ull) {
-        bword = w.l0[bk];
+      if (bset) {
+        bword = w.l0[lf];
         if (kStats) l0f++;
       }
     }
     return (uint32_t)(bword >> (code & 63u)) & 1u;
   }
   __device__ __forceinline__ uint32_t material(const DenseWorld& w, uint32_t code, uint32_t,
-                                               uint32_t& leaf) {
-    leaf++;
+                                               uint32_t& leaf_reads) {
+    leaf_reads++;
     return w.vox[code];
   }
 };
+using BitmaskLookup = BitmaskLookupT<4>;
+using BitmaskLookup8 = BitmaskLookupT<8>;
 
 // Result of one traceMap call in a dense world.
 struct DenseHit {

@@ -1,10 +1,11 @@
 // vt_world.hip -- device-side construction of dense world layouts.
 //
 // The caller hands over a linear uint8 grid (x fastest).  On the device it is
-// re-laid in Morton order (voxel (x,y,z) at interleave(x,y,z)), and the two
+// re-laid in Morton order (voxel (x,y,z) at interleave(x,y,z)), and the
 // occupancy levels are built from it: L0 = one uint64 per 64 consecutive
 // codes (a 4^3 brick, bit = code & 63), L1 = one uint64 per 4096 codes (a
-// 16^3 region, bit = brick index & 63 set when the brick has a solid voxel).
+// 16^3 region, bit = brick index & 63 set when the brick has a solid voxel),
+// L1b = one uint64 per 32768 codes (a 32^3 region, bit per 8^3 brick).
 #include "vt_internal.hpp"
 
 namespace vt {
@@ -55,6 +56,33 @@ __global__ __launch_bounds__(256) void k_build_l1(const uint64_t* __restrict__ l
   for (uint32_t j = 0; j < 64; j++)
     if (l0[i * 64u + j]) mask |= 1ull << j;
   l1[i] = mask;
+}
+
+// The 8^3-brick level: one uint64 per 32^3 region (32768 codes), bit j set
+// when 8^3 brick 64 i + j (L0 words 8 (64 i + j) .. + 7) holds a solid
+// voxel.  A 16^3 world has one partial word (8 bricks).  One wave per word.
+__global__ __launch_bounds__(256) void k_build_l1b(const uint64_t* __restrict__ l0, uint64_t n0,
+                                                   const uint32_t* __restrict__ words,
+                                                   uint32_t nwords, uint64_t* __restrict__ l1b) {
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t lane = threadIdx.x & 63u;
+  if (wave >= nwords) return;  // whole waves leave together
+  const uint32_t word = words ? words[wave] : wave;
+  const uint64_t first = ((uint64_t)word * 64u + lane) * 8u;
+  uint64_t any = 0;
+  if (first < n0)
+#pragma unroll
+    for (uint32_t k = 0; k < 8; k++) any |= l0[first + k];
+  const uint64_t m = __ballot(any != 0ull);
+  if (lane == 0) l1b[word] = m;
+}
+
+hipError_t launch_build_l1b(const uint64_t* l0, uint64_t n0, const uint32_t* words,
+                            uint32_t nwords, uint64_t* l1b, hipStream_t stream) {
+  if (nwords == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_build_l1b, dim3((nwords + 3u) / 4u), dim3(256), 0, stream, l0, n0, words,
+                     nwords, l1b);
+  return hipGetLastError();
 }
 
 // ---- compact albedo index: exclusive prefix sum of popcount(L0 word) ----
@@ -151,7 +179,7 @@ hipError_t launch_compact_prefix(const uint64_t* l0, uint64_t nwords, uint32_t* 
 }
 
 hipError_t launch_build_dense(const uint8_t* linear, uint32_t dim, uint8_t* vox, uint64_t* l0,
-                              uint64_t* l1, hipStream_t stream) {
+                              uint64_t* l1, uint64_t* l1b, hipStream_t stream) {
   const uint64_t n = (uint64_t)dim * dim * dim;  // a power of two >= 4096
   const uint64_t n0 = n / 64u, n1 = n / 4096u;
   hipLaunchKernelGGL(k_build_morton, dim3((uint32_t)((n / 4u + 255) / 256)), dim3(256), 0, stream,
@@ -164,7 +192,9 @@ hipError_t launch_build_dense(const uint8_t* linear, uint32_t dim, uint8_t* vox,
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_build_l1, dim3((uint32_t)((n1 + 255) / 256)), dim3(256), 0, stream, l0, n1,
                      l1);
-  return hipGetLastError();
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_build_l1b(l0, n0, nullptr, (uint32_t)l1b_words(dim), l1b, stream);
 }
 
 // One thread per block of the layers y < y_end of a brickmap world (linear

@@ -1,7 +1,7 @@
 """MI355X voxel ray traversal: Python host side over libvt.so (include/vt.h)."""
 from . import _native, camera, scenes  # noqa: F401
 from ._native import (VT_FLAG_GBUF_UNORM8, VT_FLAG_LAMBERT, VT_TRAVERSAL_BITMASK,  # noqa: F401
-                      VT_TRAVERSAL_FLAT, VtError, lib)
+                      VT_TRAVERSAL_BITMASK8, VT_TRAVERSAL_FLAT, VtError, lib)
 from .camera import look_at  # noqa: F401
 from ._native import VT_TILE_8X8, VT_TILE_16X4, VT_TILE_32X2  # noqa: F401
 from ._native import (VT_FLAG_ALBEDO_COMPACT, VT_FLAG_GRAD_L2, VT_FLAG_PRESENT,  # noqa: F401

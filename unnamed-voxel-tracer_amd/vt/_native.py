@@ -41,6 +41,7 @@ VT_UPSTREAM_GRADIENT = 0
 VT_UPSTREAM_L2_TARGET = 1
 VT_TRAVERSAL_FLAT = 0
 VT_TRAVERSAL_BITMASK = 1
+VT_TRAVERSAL_BITMASK8 = 2
 
 STAT_NAMES = ["primary_steps", "shadow_steps", "primary_hits", "occluded",
               "l1_fetches", "l0_fetches", "leaf_reads", "entity_hits"]

@@ -25,6 +25,60 @@ def _torch():
     return torch
 
 
+# ---- argument checks ---------------------------------------------------------
+# The C-ABI takes raw device pointers and cannot see how large the buffers
+# behind them are; these checks run before every call that launches work, so
+# a mismatched tensor raises here instead of a kernel writing past it.
+
+def _dtype_name(t):
+    return str(getattr(t, "dtype", type(t).__name__)).replace("torch.", "")
+
+
+def check_tensor(t, name, dtype, min_numel, device):
+    """t must be a contiguous tensor on HIP device `device`, of `dtype` (a
+    dtype name: "float32", "int32", "int64") with at least min_numel
+    elements.  None passes (the C-ABI treats it as absent)."""
+    if t is None:
+        return
+    if not getattr(t, "is_cuda", False):
+        raise ValueError(f"{name}: needs a tensor on the HIP device (got {getattr(t, 'device', t)})")
+    if t.device.index != device:
+        raise ValueError(f"{name}: on cuda:{t.device.index}, the context is on cuda:{device}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: must be contiguous")
+    if _dtype_name(t) != dtype:
+        raise TypeError(f"{name}: dtype {_dtype_name(t)}, expected {dtype}")
+    if t.numel() < min_numel:
+        raise ValueError(f"{name}: {t.numel()} elements, needs at least {min_numel}")
+
+
+def albedo_len(world, flags) -> int:
+    """Floats an albedo (or its gradient) holds for this world and flags."""
+    if flags & N.VT_FLAG_ALBEDO_COMPACT:
+        if world.n_occupied is None:
+            raise ValueError("VT_FLAG_ALBEDO_COMPACT: build the compact index first "
+                             "(build_compact_index)")
+        return 3 * world.n_occupied
+    return 3 * world.dim ** 3
+
+
+def check_params(ctx, world, prm, who="render"):
+    """The device buffers params() recorded, against the world and flags."""
+    keep = getattr(prm, "_tensors", None)
+    if keep is None:
+        if prm.albedo or prm.grad_albedo or prm.grad_sun or prm.loss:
+            raise ValueError(f"{who}: RenderParams with device buffers must come from vt.params()")
+        return
+    dev = ctx.device
+    lam = bool(prm.flags & N.VT_FLAG_LAMBERT)
+    if lam or keep["albedo"] is not None or keep["grad_albedo"] is not None:
+        n = albedo_len(world, prm.flags) if world.kind == "dense" else 0
+        check_tensor(keep["albedo"], f"{who}: albedo", "float32", n, dev)
+        check_tensor(keep["grad_albedo"], f"{who}: grad_albedo", "float32", n, dev)
+    check_tensor(keep["grad_sun"], f"{who}: grad_sun", "float32", 3, dev)
+    check_tensor(keep["loss"], f"{who}: loss", "float32", 1, dev)
+
+
 class Context:
     def __init__(self, device: int = 0):
         self.device = device
@@ -50,9 +104,23 @@ class Context:
             pass
 
 
+def check_brickmap_arrays(dim, chunks, pool, atlas):
+    """Sizes vt_world_create_brickmap reads from the host arrays."""
+    if dim < 8 or dim % 8:
+        raise ValueError(f"brickmap: dim {dim} must be a positive multiple of 8")
+    if chunks.size != (dim // 8) ** 3:
+        raise ValueError(f"brickmap: {chunks.size} chunk entries, a {dim}^3 map has "
+                         f"{(dim // 8) ** 3}")
+    if pool.size % 512:
+        raise ValueError(f"brickmap: pool of {pool.size} words is not whole 8^3 blocks")
+    if atlas.size != 256 ** 3:
+        raise ValueError(f"brickmap: atlas of {atlas.size} texels, expected 256^3")
+
+
 class World:
     def __init__(self, ctx: Context, h: C.c_void_p, dim: int, kind: str):
         self.ctx, self.h, self.dim, self.kind = ctx, h, dim, kind
+        self.n_occupied = None  # compact index size, once built
 
     @classmethod
     def dense(cls, ctx: Context, grid: np.ndarray, palette: np.ndarray) -> "World":
@@ -69,9 +137,10 @@ class World:
     @classmethod
     def brickmap(cls, ctx: Context, dim: int, chunks: np.ndarray, pool: np.ndarray,
                  atlas: np.ndarray) -> "World":
-        chunks = np.ascontiguousarray(chunks, dtype=np.uint32)
-        pool = np.ascontiguousarray(pool, dtype=np.uint32)
-        atlas = np.ascontiguousarray(atlas, dtype=np.uint32)
+        chunks = np.ascontiguousarray(chunks, dtype=np.uint32).reshape(-1)
+        pool = np.ascontiguousarray(pool, dtype=np.uint32).reshape(-1)
+        atlas = np.ascontiguousarray(atlas, dtype=np.uint32).reshape(-1)
+        check_brickmap_arrays(dim, chunks, pool, atlas)
         h = C.c_void_p()
         N.check(N.lib().vt_world_create_brickmap(ctx.h, dim, chunks.ctypes.data,
                                                  pool.ctypes.data if pool.size else None,
@@ -93,6 +162,8 @@ class World:
         N.check(N.lib().vt_world_set_voxels(self.ctx.h, self.h, xyz.ctypes.data,
                                             values.ctypes.data, len(values),
                                             C.c_void_p(_stream(stream))), "vt_world_set_voxels")
+        if self.n_occupied is not None:  # edits renumber the compact index
+            build_compact_index(self.ctx, self)
 
     def get_voxels(self, xyz, stream=None) -> np.ndarray:
         xyz = np.ascontiguousarray(xyz, dtype=np.uint32).reshape(-1, 3)
@@ -184,6 +255,16 @@ class Frame:
             f.display = torch.empty((height, width), dtype=torch.int32, device=dev)
         return f
 
+    def check(self, device, who="frame"):
+        """Every buffer large enough for width x height, on `device`."""
+        n = self.width * self.height
+        for name in ("shaded", "color", "normal", "position", "illumination", "target"):
+            check_tensor(getattr(self, name), f"{who}.{name}", "float32", 4 * n, device)
+        check_tensor(self.hits, f"{who}.hits", "int32", 3 * n, device)
+        check_tensor(self.steps, f"{who}.steps", "int32", n, device)
+        check_tensor(self.display, f"{who}.display", "int32", n, device)
+        check_tensor(self.stats, f"{who}.stats", "int64", 8, device)
+
     def struct(self) -> N.Frame:
         p = lambda t: None if t is None else t.data_ptr()
         return N.Frame(p(self.shaded), p(self.hits), p(self.color), p(self.normal),
@@ -206,12 +287,17 @@ def params(width, height, max_primary=512, max_shadow=512, sun=SUN, flags=0, tra
     ptr = lambda t: None if t is None else t.data_ptr()
     p.albedo, p.grad_albedo, p.grad_sun, p.loss = (ptr(albedo), ptr(grad_albedo), ptr(grad_sun),
                                                    ptr(loss))
+    # the tensors behind the pointers: kept alive and checked before launches
+    p._tensors = dict(albedo=albedo, grad_albedo=grad_albedo, grad_sun=grad_sun, loss=loss)
     return p
 
 
 def present(ctx: Context, composite, out=None, display=None, stream=None):
     """Present pass (crosshair + vignette) over an (H, W, 4) float32 composite."""
     H, W = composite.shape[0], composite.shape[1]
+    check_tensor(composite, "present: composite", "float32", 4 * W * H, ctx.device)
+    check_tensor(out, "present: out", "float32", 4 * W * H, ctx.device)
+    check_tensor(display, "present: display", "int32", W * H, ctx.device)
     s = current_stream() if stream is None else stream
     ptr = lambda t: None if t is None else t.data_ptr()
     N.check(N.lib().vt_present(ctx.h, composite.data_ptr(), W, H, ptr(out), ptr(display),
@@ -222,11 +308,25 @@ def build_compact_index(ctx: Context, world: World) -> int:
     n = C.c_uint64()
     N.check(N.lib().vt_world_build_compact_index(ctx.h, world.h, C.byref(n)),
             "vt_world_build_compact_index")
+    world.n_occupied = n.value
     return n.value
+
+
+def _records(ctx, hits, who) -> int:
+    """Hit records: an int32 [..., 3] tensor; returns their count."""
+    if hits is None:
+        raise ValueError(f"{who}: hits required")
+    if hits.numel() % 3:
+        raise ValueError(f"{who}: hits has {hits.numel()} words, not whole 3-word records")
+    check_tensor(hits, f"{who}: hits", "int32", 3, ctx.device)
+    return hits.numel() // 3
 
 
 def lambert_shade(ctx: Context, world: World, hits, prm: N.RenderParams, out, stream=None):
     """out[i] = Lambert shading of hit record i (hits: int32 [..., 3] tensor)."""
+    npix = _records(ctx, hits, "lambert_shade")
+    check_params(ctx, world, prm, "lambert_shade")
+    check_tensor(out, "lambert_shade: out", "float32", 4 * npix, ctx.device)
     s = current_stream() if stream is None else stream
     N.check(N.lib().vt_lambert_shade(ctx.h, world.h, hits.data_ptr(), hits.numel() // 3,
                                      C.byref(prm), out.data_ptr(), C.c_void_p(s)),
@@ -236,6 +336,13 @@ def lambert_shade(ctx: Context, world: World, hits, prm: N.RenderParams, out, st
 
 def lambert_backward(ctx: Context, world: World, hits, prm: N.RenderParams, upstream, kind,
                      grad_albedo, grad_sun=None, loss=None, stream=None):
+    npix = _records(ctx, hits, "lambert_backward")
+    check_params(ctx, world, prm, "lambert_backward")
+    check_tensor(upstream, "lambert_backward: upstream", "float32", 4 * npix, ctx.device)
+    check_tensor(grad_albedo, "lambert_backward: grad_albedo", "float32",
+                 albedo_len(world, prm.flags), ctx.device)
+    check_tensor(grad_sun, "lambert_backward: grad_sun", "float32", 3, ctx.device)
+    check_tensor(loss, "lambert_backward: loss", "float32", 1, ctx.device)
     s = current_stream() if stream is None else stream
     ptr = lambda t: None if t is None else t.data_ptr()
     N.check(N.lib().vt_lambert_backward(ctx.h, world.h, hits.data_ptr(), hits.numel() // 3,
@@ -255,6 +362,9 @@ class Batch:
     def __init__(self, ctx: Context, world: World, views, prm: N.RenderParams,
                  tune: N.Tuning, stats=None):
         self.ctx, self.world, self.frames = ctx, world, [f for _, f in views]
+        _check_views(ctx, world, views, prm, "Batch")
+        check_tensor(stats, "Batch: stats", "int64", 8, ctx.device)
+        self.prm = prm  # keeps the albedo / gradient tensors alive
         self.table = (N.View * len(views))()
         for i, (cam, f) in enumerate(views):
             self.table[i].camera = cam
@@ -291,7 +401,14 @@ class Batch:
             pass
 
 
+def _check_views(ctx, world, views, prm, who):
+    check_params(ctx, world, prm, who)
+    for i, (_, f) in enumerate(views):
+        f.check(ctx.device, f"{who}: view {i}")
+
+
 def autotune(ctx: Context, world: World, views, prm: N.RenderParams, reps=3):
+    _check_views(ctx, world, views, prm, "autotune")
     table = (N.View * len(views))()
     for i, (cam, f) in enumerate(views):
         table[i].camera = cam
@@ -313,6 +430,11 @@ def _stream(stream):
 
 def render(ctx: Context, world: World, cam: N.Camera, frame: Frame, prm: N.RenderParams,
            stream=None) -> Frame:
+    if (prm.width, prm.height) != (frame.width, frame.height):
+        raise ValueError(f"render: params are {prm.width}x{prm.height}, the frame "
+                         f"{frame.width}x{frame.height}")
+    frame.check(ctx.device, "render: frame")
+    check_params(ctx, world, prm, "render")
     fs = frame.struct()
     s = current_stream() if stream is None else stream
     N.check(N.lib().vt_render(ctx.h, world.h, C.byref(cam), C.byref(prm), C.byref(fs),

@@ -142,3 +142,55 @@ def views(name: str):
 
 
 MAX_STEPS = {"c0": 512, "c1": 2048, "c2": 2048, "c3": 1024, "c4": 2048}
+
+
+def _part1by2(t):
+    t = t & 0x3FF
+    t = (t | (t << 16)) & 0xFF0000FF
+    t = (t | (t << 8)) & 0x0300F00F
+    t = (t | (t << 4)) & 0x030C30C3
+    return (t | (t << 2)) & 0x09249249
+
+
+def morton_codes(vox, dim):
+    """Morton code of linear voxel ids x + dim*(y + dim*z) (int64 tensor)."""
+    lg = dim.bit_length() - 1
+    m = dim - 1
+    return (_part1by2(vox & m) | (_part1by2((vox >> lg) & m) << 1) |
+            (_part1by2(vox >> (2 * lg)) << 2))
+
+
+def occupied_morton(grid):
+    """(occupied voxel ids in Morton order, their sorted Morton codes), on
+    the GPU: the order of the library's compact albedo index."""
+    import torch
+    g = torch.from_numpy(grid.reshape(-1)).cuda()
+    occ = torch.nonzero(g).squeeze(1)
+    del g
+    code, order = torch.sort(morton_codes(occ, grid.shape[0]))
+    return occ[order], code
+
+
+def ground_truth_albedo(grid, pal, compact=False, seed=0x5EED):
+    """The differentiable workloads' ground truth: material base colour +
+    N(0, 0.05) per voxel and channel, float32 RGB on the GPU.
+
+    dense:   dim^3 x 3, indexed by voxel id x + dim*(y + dim*z);
+    compact: n_occupied x 3, occupied voxels in Morton order (the order of
+             the library's compact index)."""
+    import torch
+    base = torch.tensor([[(int(p) >> s) & 0xFF for s in (0, 8, 16)] for p in pal],
+                        dtype=torch.float32, device="cuda") / 255.0
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    if compact:
+        occ, _ = occupied_morton(grid)
+        g = torch.from_numpy(grid.reshape(-1)).cuda()
+        mats = g[occ].long()
+        del occ, g
+    else:
+        mats = torch.from_numpy(grid.reshape(-1)).cuda().long()
+    a = base[mats]
+    del mats
+    a += 0.05 * torch.randn(a.shape, generator=gen, device="cuda")
+    return a.reshape(-1).contiguous()
