@@ -376,6 +376,23 @@ int vt_batch_autotune(vt_ctx* ctx, const vt_world* world, const vt_view* views,
                       uint32_t n_views, const vt_render_params* params, int reps,
                       vt_tuning* best, float* best_ms);
 
+/* ---- batched rendering without allocation ------------------------------- */
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  Renders the views like vt_batch_create + vt_batch_launch (same
+ * kernels, same outputs bit for bit) but allocates nothing: the view table
+ * and the work-queue counters live in a fixed device array of the library
+ * (about 45 KiB per GPU, held until the process ends), whose address each
+ * vt_create takes once per device.  The view records are handed over as
+ * kernel arguments, so `views` may be freed when the call returns.  At most
+ * VT_NOALLOC_MAX_VIEWS views (else VT_ERR_INVALID_ARGUMENT).  All calls on
+ * one GPU share that array: calls that overlap on one GPU (other streams,
+ * other threads or contexts) race; order them on one stream.             */
+#define VT_NOALLOC_MAX_VIEWS 256u
+int vt_batch_render_noalloc(vt_ctx* ctx, const vt_world* world, const vt_view* views,
+                            uint32_t n_views, const vt_render_params* params,
+                            const vt_tuning* tuning, unsigned long long* stats, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -281,6 +281,12 @@ int vt_create(vt_ctx** out, int device) {
     return fail(VT_ERR_OUT_OF_MEMORY, "vt_create: %zu bytes of staging: %s", kCtxStageBytes,
                 hipGetErrorString(e));
   }
+  int st = vt::noalloc_init(device);
+  if (st != VT_OK) {
+    dev_free(c->stage);
+    delete c;
+    return st;
+  }
   *out = c;
   return VT_OK;
 }

@@ -10,6 +10,7 @@
 // drains.  The per-pixel work is trace_pixel_dense, shared with vt_render.
 #include <chrono>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -83,6 +84,59 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     flush_counters(o, cnt);
   }
   if constexpr (kShade == kShadeLambertGrad) flush_grad(sc, gacc);
+}
+
+// ---- the no-allocation batch (vt_batch_render_noalloc) ----------------------
+// Its view table and queue counters live in a file-scope device array (one
+// copy per GPU, held until the process ends) instead of memory the library
+// allocates.  Host code never passes the array's name to the device: its
+// address on each device comes from hipGetSymbolAddress, taken once per
+// device at vt_create (noalloc_init) with that device current.
+struct NoallocScratch {
+  BatchView views[VT_NOALLOC_MAX_VIEWS];
+  uint32_t queues[kQueues];
+};
+
+static __device__ __align__(256) NoallocScratch scratch;
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static NoallocScratch* g_scratch_ptr[kMaxDevices];  // per device ordinal, null until set
+
+int noalloc_init(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(VT_ERR_NO_DEVICE, "vt_create: device %d beyond the %d this library tracks",
+                device, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return VT_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                    "vt_create: address of the no-allocation scratch");
+  }
+  g_scratch_ptr[device] = static_cast<NoallocScratch*>(p);
+  return VT_OK;
+}
+
+static NoallocScratch* noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return (device >= 0 && device < kMaxDevices) ? g_scratch_ptr[device] : nullptr;
+}
+
+// Views handed to the device as kernel arguments (copied by the launch
+// itself, so the host table may go away at once), written into the scratch
+// view table: up to kStageViews per launch.
+constexpr uint32_t kStageViews = 8;
+struct StagePack {
+  BatchView v[kStageViews];
+  uint32_t first, count;
+};
+
+__global__ void k_stage_views(BatchView* __restrict__ dst, StagePack pack) {
+  const uint32_t i = threadIdx.x;
+  if (i < pack.count) dst[pack.first + i] = pack.v[i];
 }
 
 }  // namespace vt
@@ -348,6 +402,39 @@ done:
   if (st != VT_OK) return st == VT_ERR_DEVICE ? fail(st, "vt_batch_autotune: timing") : st;
   *best = best_cfg;
   *best_ms = best_t;
+  return VT_OK;
+}
+
+int vt_batch_render_noalloc(vt_ctx* ctx, const vt_world* world, const vt_view* views,
+                            uint32_t n_views, const vt_render_params* params,
+                            const vt_tuning* tuning, unsigned long long* stats, void* stream) {
+  vt_render_params p;
+  int st = check_batch(ctx, world, views, n_views, params, tuning, stats, VT_NOALLOC_MAX_VIEWS,
+                       "vt_batch_render_noalloc", &p);
+  if (st != VT_OK) return st;
+  VT_HIP(hipSetDevice(ctx->device));
+  vt::NoallocScratch* sp = vt::noalloc_scratch(ctx->device);
+  if (!sp) return fail(VT_ERR_DEVICE, "vt_batch_render_noalloc: no scratch address for device %d",
+                       ctx->device);
+  const hipStream_t s = (hipStream_t)stream;
+  const vt_tuning t = *tuning;
+  // view records, kStageViews per staging launch (kernel arguments)
+  uint64_t tiles = 0;
+  vt::StagePack pack;
+  for (uint32_t i0 = 0; i0 < n_views; i0 += vt::kStageViews) {
+    pack.first = i0;
+    pack.count = std::min(n_views - i0, vt::kStageViews);
+    for (uint32_t k = 0; k < pack.count; k++)
+      if ((st = view_record(views[i0 + k], t.tile, &tiles, &pack.v[k],
+                            "vt_batch_render_noalloc")) != VT_OK)
+        return st;
+    hipLaunchKernelGGL(vt::k_stage_views, dim3(1), dim3(64), 0, s, sp->views, pack);
+    VT_HIP(hipGetLastError());
+  }
+  const hipError_t e = launch_batch(world, vt::shade_consts(p), sp->views, n_views,
+                                    (uint32_t)tiles, sp->queues, t,
+                                    (uint32_t)ctx->compute_units * t.blocks_per_cu, stats, s);
+  if (e != hipSuccess) return hip_fail(e, "vt_batch_render_noalloc: launch");
   return VT_OK;
 }
 

@@ -100,6 +100,10 @@ struct TileShape {
 DenseWorld dense_view(const vt_world* w);
 BrickmapWorld brickmap_view(const vt_world* w);
 
+// Takes the device address of the no-allocation scratch array on `device`
+// (the current device) once per device; called by vt_create.
+int noalloc_init(int device);
+
 // Host-side validation and translation of the C-ABI structs.
 int check_image_size(uint32_t W, uint32_t H, const char* who);
 int check_params(const vt_render_params& p, const vt_world* world, const char* who);

@@ -85,6 +85,7 @@ class PickResult(C.Structure):
 
 
 VT_TILE_8X8, VT_TILE_16X4, VT_TILE_32X2 = 3, 4, 5
+VT_NOALLOC_MAX_VIEWS = 256
 
 assert C.sizeof(Camera) == 96
 assert C.sizeof(RenderParams) == 72
@@ -145,6 +146,8 @@ def lib() -> C.CDLL:
         "vt_batch_destroy": (i32, [vp]),
         "vt_batch_autotune": (i32, [vp, vp, C.POINTER(View), u32, C.POINTER(RenderParams),
                                     i32, C.POINTER(Tuning), C.POINTER(C.c_float)]),
+        "vt_batch_render_noalloc": (i32, [vp, vp, C.POINTER(View), u32,
+                                          C.POINTER(RenderParams), C.POINTER(Tuning), vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
