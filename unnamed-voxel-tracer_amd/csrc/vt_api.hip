@@ -207,6 +207,8 @@ ViewConsts view_consts(const vt_camera* camera, uint32_t W, uint32_t H) {
 ShadeConsts shade_consts(const vt_render_params& p) {
   ShadeConsts sc;
   std::memcpy(sc.sun, p.sun, sizeof sc.sun);
+  // the dense launchers redo this with the world's axis masks
+  sc.shadow = make_ray_consts(p.sun[0], p.sun[1], p.sun[2], 0u, 0u, 0u);
   sc.max_primary = p.max_steps_primary;
   sc.max_shadow = p.max_steps_shadow;
   sc.flags = p.flags;

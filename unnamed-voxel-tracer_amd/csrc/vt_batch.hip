@@ -35,8 +35,11 @@ __global__ void k_batch_reset(uint32_t* __restrict__ queues) {
 // VGPRs (5 waves); capping it at 64 spills a few values that live outside the
 // traversal loops and hides the gathers' latency far better (C2: 19.5 ->
 // 16.3 ms per 64-view batch).
+#ifndef VT_BATCH_WAVES_PER_SIMD
+#define VT_BATCH_WAVES_PER_SIMD 8
+#endif
 template <class L, bool kStats, int kShade>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_render_batch(DenseWorld w, ShadeConsts sc,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VT_BATCH_WAVES_PER_SIMD))) void k_render_batch(DenseWorld w, ShadeConsts sc,
                                                       const BatchView* __restrict__ views,
                                                       uint32_t n_views, uint32_t total_tiles,
                                                       uint32_t* __restrict__ queues,
@@ -161,11 +164,13 @@ using vt::fail;
 using vt::hip_fail;
 
 // Queue reset + persistent launch over a device view table.
-hipError_t launch_batch(const vt_world* world, const vt::ShadeConsts& sc,
+hipError_t launch_batch(const vt_world* world, const vt::ShadeConsts& sc_in,
                         const vt::BatchView* d_views, uint32_t n_views, uint32_t total_tiles,
                         uint32_t* d_queues, const vt_tuning& tuning, uint32_t nblocks,
                         unsigned long long* stats, hipStream_t s) {
   const vt::DenseWorld w = vt::dense_view(world);
+  vt::ShadeConsts sc = sc_in;
+  sc.shadow = vt::make_ray_consts(sc.sun[0], sc.sun[1], sc.sun[2], w.mx, w.my, w.mz);
   hipLaunchKernelGGL(vt::k_batch_reset, dim3(1), dim3(64), 0, s, d_queues);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;

@@ -68,6 +68,7 @@ void dev_free(void* p);
 // Per-launch shading constants.
 struct ShadeConsts {
   float sun[3];
+  RayConsts shadow;  // the shadow rays' direction constants (set by the dense launchers)
   int max_primary, max_shadow;
   uint32_t flags;
   float ambient;

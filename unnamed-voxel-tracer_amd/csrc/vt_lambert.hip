@@ -47,48 +47,81 @@ __global__ __launch_bounds__(256) void k_lambert_shade(DenseWorld w, ShadeConsts
   out[i] = s;
 }
 
+// Pixels per lane per block iteration of the backward: the loads of all
+// of them (records, upstream, then the albedo gathers of the hits) are
+// issued before any is used, so each lane keeps that many dependent gather
+// chains in flight instead of one.
+constexpr uint32_t kBwdUnroll = 4;
+
 __global__ __launch_bounds__(256) void k_lambert_backward(DenseWorld w, ShadeConsts sc,
                                                           const uint32_t* __restrict__ hits,
                                                           uint32_t npix,
                                                           const float4* __restrict__ up,
                                                           uint32_t kind) {
   GradAcc acc = grad_acc();
-  // grid-stride over 256-pixel blocks (the bound is block-uniform, so every
-  // wave reaches each flush_albedo); the launch caps the grid so that the
-  // per-block loss / sun atomics stay few
-  for (uint32_t base = blockIdx.x * 256u; base < npix; base += gridDim.x * 256u) {
-    const uint32_t i = base + threadIdx.x;
-    if (i < npix) {
-      const uint3 rec = load_record(hits, i);
-      const uint32_t voxel = rec.x, info = rec.z;
-      const float4 u = up[i];
-      if (record_hit(w, voxel)) {
+  const LambertSun ls = lambert_sun(sc.sun);
+  const bool compact = (sc.flags & VT_FLAG_ALBEDO_COMPACT) != 0u;
+  const uint32_t span = 256u * kBwdUnroll;
+  // grid-stride over block spans of 256 x kBwdUnroll pixels, the wave's
+  // lanes on 64 consecutive pixels per sub-span (the bound is block-uniform,
+  // so every wave reaches each flush_albedo); the launch caps the grid so
+  // that the per-block loss / sun atomics stay few
+  for (uint32_t base = blockIdx.x * span; base < npix; base += gridDim.x * span) {
+    uint3 rec[kBwdUnroll];
+    float4 u[kBwdUnroll];
+#pragma unroll
+    for (uint32_t j = 0; j < kBwdUnroll; j++) {
+      const uint32_t i = base + j * 256u + threadIdx.x;
+      rec[j] = make_uint3(0xFFFFFFFFu, 0u, 0u);
+      u[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (i < npix) {
+        rec[j] = load_record(hits, i);
+        u[j] = up[i];
+      }
+    }
+    uint32_t ai[kBwdUnroll];
+    float A[kBwdUnroll][3];
+#pragma unroll
+    for (uint32_t j = 0; j < kBwdUnroll; j++) {
+      ai[j] = kNoKey;
+      A[j][0] = A[j][1] = A[j][2] = 0.0f;
+      if (record_hit(w, rec[j].x)) {
+        const uint32_t code = compact ? voxel_code(w, rec[j].x) : 0u;
+        ai[j] = albedo_index(w, sc.flags, rec[j].x, code);
+        const float* a = sc.albedo + 3 * (size_t)ai[j];
+        A[j][0] = a[0];
+        A[j][1] = a[1];
+        A[j][2] = a[2];
+      }
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kBwdUnroll; j++) {
+      if (ai[j] != kNoKey) {
+        const uint32_t info = rec[j].z;
         const int face = (int)(info & 7u);
         const bool occl = ((info >> 3) & 1u) != 0u;
-        const LambertSun ls = lambert_sun(sc.sun);
         const float nd = lambert_nd(face, ls);
         const float fct = lambert_factor(sc.ambient, nd, occl);
-        const uint32_t code = (sc.flags & VT_FLAG_ALBEDO_COMPACT) ? voxel_code(w, voxel) : 0u;
-        const uint32_t ai = albedo_index(w, sc.flags, voxel, code);
-        const float* a = sc.albedo + 3 * (size_t)ai;
         float g[3];
         if (kind == VT_UPSTREAM_L2_TARGET) {
-          const float r[3] = {a[0] * fct - u.x, a[1] * fct - u.y, a[2] * fct - u.z};
+          const float r[3] = {A[j][0] * fct - u[j].x, A[j][1] * fct - u[j].y,
+                              A[j][2] * fct - u[j].z};
           acc.loss += r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
           g[0] = 2.0f * r[0];
           g[1] = 2.0f * r[1];
           g[2] = 2.0f * r[2];
         } else {
-          g[0] = u.x;
-          g[1] = u.y;
-          g[2] = u.z;
+          g[0] = u[j].x;
+          g[1] = u[j].y;
+          g[2] = u[j].z;
         }
-        lambert_accumulate(ai, face, nd, occl, a, g, fct, ls, acc);
+        lambert_accumulate(ai[j], face, nd, occl, A[j], g, fct, ls, acc);
       } else if (kind == VT_UPSTREAM_L2_TARGET) {
-        acc.loss += u.x * u.x + u.y * u.y + u.z * u.z;
+        // sky pixel (or past the end: u = 0 adds nothing)
+        acc.loss += u[j].x * u[j].x + u[j].y * u[j].y + u[j].z * u[j].z;
       }
+      flush_albedo(sc, acc);
     }
-    flush_albedo(sc, acc);
   }
   flush_grad(sc, acc);
 }
@@ -103,7 +136,8 @@ hipError_t launch_lambert_shade(const DenseWorld& w, const ShadeConsts& sc, cons
 hipError_t launch_lambert_backward(const DenseWorld& w, const ShadeConsts& sc,
                                    const uint32_t* hits, uint32_t npix, const float4* upstream,
                                    uint32_t kind, hipStream_t stream) {
-  const uint32_t blocks = std::min<uint32_t>((npix + 255) / 256, 4096u);
+  const uint32_t span = 256u * kBwdUnroll;
+  const uint32_t blocks = std::min<uint32_t>((npix + span - 1) / span, 4096u);
   hipLaunchKernelGGL(k_lambert_backward, dim3(blocks), dim3(256), 0, stream, w, sc, hits, npix,
                      upstream, kind);
   return hipGetLastError();

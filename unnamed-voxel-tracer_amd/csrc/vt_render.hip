@@ -317,8 +317,10 @@ static void launch_dense_shade(const DenseWorld& w, const ViewConsts& v, const S
 }
 
 hipError_t launch_render_dense(const DenseWorld& w, uint32_t traversal, const ViewConsts& v,
-                               const ShadeConsts& sc, const FrameOut& out, TileShape tile,
+                               const ShadeConsts& sc_in, const FrameOut& out, TileShape tile,
                                uint32_t compute_units, hipStream_t stream) {
+  ShadeConsts sc = sc_in;
+  sc.shadow = make_ray_consts(sc.sun[0], sc.sun[1], sc.sun[2], w.mx, w.my, w.mz);
   uint32_t blocks_x, nblocks;
   grid_shape(v, tile, blocks_x, nblocks);
   const bool pack = nblocks >= VT_PACK_BLOCKS_PER_CU * compute_units &&

@@ -337,42 +337,88 @@ struct DenseRay {
 
 // The per-ray constants of traceMap's set-up (map.glsl:85-97): the 0.001
 // substitution of zero direction components, positivity, 1/d, and the
-// block-level crossing and re-anchoring constants.
+// block-level crossing and re-anchoring constants.  A __host__ __device__
+// function: the shadow rays' direction is the sun for every pixel, so their
+// constants are computed once per launch on the host (binary32, correctly
+// rounded division, like the device) and reach the kernels as arguments --
+// scalar registers instead of per-lane copies of uniform values.
+struct RayConsts {
+  float d[3];    // rayDir after the zero substitution
+  float inv[3];  // 1 / rayDir
+  float P[3];    // float(rayPositivity << 3)
+  float A[3];    // float((1 - rayPositivity) << 3) * 0.999f
+  int pos[3];    // rayPositivity
+  uint32_t flip; // masks of the axes the ray travels negatively
+};
+
+__host__ __device__ inline RayConsts make_ray_consts(float dx, float dy, float dz, uint32_t mx,
+                                                     uint32_t my, uint32_t mz) {
+  RayConsts k;
+  const float d[3] = {dx == 0.0f ? 0.001f : dx, dy == 0.0f ? 0.001f : dy,
+                      dz == 0.0f ? 0.001f : dz};
+  const uint32_t m[3] = {mx, my, mz};
+  k.flip = 0u;
+  for (int i = 0; i < 3; i++) {
+    const int sg = d[i] > 0.0f ? 1 : (d[i] < 0.0f ? -1 : 0);
+    k.d[i] = d[i];
+    k.pos[i] = (1 + sg) >> 1;
+    k.inv[i] = 1.0f / d[i];
+    k.P[i] = (float)(k.pos[i] << 3);
+    k.A[i] = (float)((1 - k.pos[i]) << 3) * 0.999f;
+    k.flip |= k.pos[i] ? 0u : m[i];
+  }
+  return k;
+}
+
+__device__ __forceinline__ void dense_ray_from(DenseRay& r, const RayConsts& k) {
+  r.dx = k.d[0];
+  r.dy = k.d[1];
+  r.dz = k.d[2];
+  r.ix = k.inv[0];
+  r.iy = k.inv[1];
+  r.iz = k.inv[2];
+  r.Px = k.P[0];
+  r.Py = k.P[1];
+  r.Pz = k.P[2];
+  r.Ax = k.A[0];
+  r.Ay = k.A[1];
+  r.Az = k.A[2];
+  r.px = k.pos[0];
+  r.py = k.pos[1];
+  r.pz = k.pos[2];
+  r.flip = k.flip;
+}
+
 __device__ __forceinline__ void dense_ray_consts(DenseRay& r, const DenseWorld& w, float dx,
                                                  float dy, float dz) {
-  if (dx == 0.0f) dx = 0.001f;
-  if (dy == 0.0f) dy = 0.001f;
-  if (dz == 0.0f) dz = 0.001f;
-  r.dx = dx;
-  r.dy = dy;
-  r.dz = dz;
-  const int sx = dx > 0.0f ? 1 : (dx < 0.0f ? -1 : 0);
-  const int sy = dy > 0.0f ? 1 : (dy < 0.0f ? -1 : 0);
-  const int sz = dz > 0.0f ? 1 : (dz < 0.0f ? -1 : 0);
-  r.px = (1 + sx) >> 1;
-  r.py = (1 + sy) >> 1;
-  r.pz = (1 + sz) >> 1;
-  r.ix = 1.0f / dx;
-  r.iy = 1.0f / dy;
-  r.iz = 1.0f / dz;
-  r.Px = (float)(r.px << 3);
-  r.Py = (float)(r.py << 3);
-  r.Pz = (float)(r.pz << 3);
-  r.Ax = (float)((1 - r.px) << 3) * 0.999f;
-  r.Ay = (float)((1 - r.py) << 3) * 0.999f;
-  r.Az = (float)((1 - r.pz) << 3) * 0.999f;
-  r.flip = (r.px ? 0u : w.mx) | (r.py ? 0u : w.my) | (r.pz ? 0u : w.mz);
+  dense_ray_from(r, make_ray_consts(dx, dy, dz, w.mx, w.my, w.mz));
 }
 
 // Set-up (map.glsl:85-104) plus iteration 0 (stepSize 0).  Returns true when
 // the ray already terminated (hit or miss) in iteration 0; otherwise the ray
 // is at the block level, ready for the DDA step of iteration 0.
+// dense_begin_k: the same with the direction's constants made beforehand.
+template <bool kStats, class L>
+__device__ __forceinline__ bool dense_begin_k(DenseRay& r, L& lk, const DenseWorld& w, float ox,
+                                              float oy, float oz, const RayConsts& k,
+                                              int max_steps, DenseHit& h, uint32_t& l1f,
+                                              uint32_t& l0f);
+
 template <bool kStats, class L>
 __device__ __forceinline__ bool dense_begin(DenseRay& r, L& lk, const DenseWorld& w, float ox,
                                             float oy, float oz, float dx, float dy, float dz,
                                             int max_steps, DenseHit& h, uint32_t& l1f,
                                             uint32_t& l0f) {
-  dense_ray_consts(r, w, dx, dy, dz);
+  return dense_begin_k<kStats>(r, lk, w, ox, oy, oz, make_ray_consts(dx, dy, dz, w.mx, w.my, w.mz),
+                               max_steps, h, l1f, l0f);
+}
+
+template <bool kStats, class L>
+__device__ __forceinline__ bool dense_begin_k(DenseRay& r, L& lk, const DenseWorld& w, float ox,
+                                              float oy, float oz, const RayConsts& k,
+                                              int max_steps, DenseHit& h, uint32_t& l1f,
+                                              uint32_t& l0f) {
+  dense_ray_from(r, k);
   r.steps = 0;
   r.max_steps = max_steps;
   lk.reset();
