@@ -24,6 +24,12 @@ sys.path.insert(0, os.path.join(ROOT, "unnamed-voxel-tracer_amd"))
 
 METRIC = ("Mrays/s primary+shadow, 1080p/512³ grid; fwd+bwd ms per view vs plain kernel")
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak: one wave64 VALU instruction per SIMD every 2 cycles at
+# 2.4 GHz on 1,024 SIMDs -- the rate behind the published 157.3 TFLOP/s FP32
+# vector peak (64 lanes x 2 flops per FMA instruction)
+VALU_PEAK_GIPS = 1024 * 2.4 / 2
+LOWER_BYTES_PER_STEP = 1  # every traceMap step reads at least one byte
+SUN_DIR = (7.52185881e-01, 6.58950984e-01, 7.52185881e-01)  # camera.glsl:8
 TRAVERSALS = ["flat", "bitmask", "bitmask8"]  # VT_TRAVERSAL_FLAT, _BITMASK, _BITMASK8
 OUT_BYTES_PER_PIXEL = 16 + 12  # RGBA32F shaded + 12-byte hit record
 
@@ -212,9 +218,13 @@ def bench_brickmap(args):
                      args.warmup)
     npix = W * H
     rays = npix + st["primary_hits"]
-    # 4 B chunk-table read per in-bounds step, 4 B pool read per step in an
-    # allocated chunk, 4 B atlas read per step in a nonempty block, + outputs
-    abytes = 4 * (st["l1_fetches"] + st["l0_fetches"]) + OUT_BYTES_PER_PIXEL * npix
+    # what the timed kernel must read at least: one byte per traceMap step
+    # (an occupancy byte in the empty-space loop, a 4-byte chunk word in the
+    # literal loop) + the outputs; the literal loop's own reads -- 4 B chunk,
+    # 4 B pool, 4 B atlas per step -- are the upper figure
+    abytes = LOWER_BYTES_PER_STEP * (st["primary_steps"] + st["shadow_steps"]) + \
+        OUT_BYTES_PER_PIXEL * npix
+    literal_bytes = 4 * (st["l1_fetches"] + st["l0_fetches"]) + OUT_BYTES_PER_PIXEL * npix
     achieved = abytes / (ms * 1e-3) / 1e9
     line = {
         "metric": "Mrays/s primary+shadow, 1080p/512³ grid in the engine's chunk/pool/atlas "
@@ -240,7 +250,9 @@ def bench_brickmap(args):
             "dda_steps": st["primary_steps"] + st["shadow_steps"],
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "bytes": "lower bound: 1 B per step + 28 B per pixel",
+                     "literal_loop_gbs": round(literal_bytes / (ms * 1e-3) / 1e9, 2)},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -497,6 +509,24 @@ def measured_traffic(key):
         return None
 
 
+def measured_issue(key, kernel_ms):
+    """The VALU issue roofline of a kernel/workload: its wave64 VALU
+    instructions per launch (rocprofv3 SQ_INSTS_VALU, profiles/traffic.json)
+    over the kernel time measured in this run, against VALU_PEAK_GIPS; the
+    lanes active per VALU instruction from SQ_THREAD_CYCLES_VALU /
+    SQ_ACTIVE_INST_VALU.  None if the workload has not been measured."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            e = json.load(f)["valu"][key]
+        gips = e["insts"] / (kernel_ms * 1e-3) / 1e9
+        return {"bound": "valu", "achieved": round(gips, 1), "peak": VALU_PEAK_GIPS,
+                "unit": "G wave64 VALU instr/s", "frac": round(gips / VALU_PEAK_GIPS, 4),
+                "lanes_active": round(e["thread_cycles"] / e["active_inst"] / 64.0, 4),
+                "valu_insts": e["insts"]}
+    except (OSError, KeyError, ValueError, ZeroDivisionError):
+        return None
+
+
 def bench_procgen(args):
     """The engine's world generator as Game.init runs it, procgen(512,
     &voxels, 0, 0) (game.zig:96-97), into a fresh empty 512^3 brickmap world
@@ -570,12 +600,15 @@ def bench_procgen(args):
 
 
 def bench_engine(args):
-    """The engine's own frame: Game.init's world (procgen(512, &voxels, 0, 0)
-    into the chunk/pool layout, game.zig:96-97) with the model atlas of its
-    .vox assets (the placed texels kept in tests/golden/atlas_assets.npz),
-    its start camera (player (256, 22, 256) + 3 up, identity rotation, fov
-    pi/2), the shaders' own step caps (192 primary, primary.comp.glsl:43;
-    48 shadow, secondary.comp.glsl:41), 1920x1080.  A step is one vt_render
+    """The engine's own frame configuration: Game.init's world
+    (procgen(512, &voxels, 0, 0) into the chunk/pool layout, game.zig:96-97)
+    with the model atlas of its .vox assets (the placed texels kept in
+    tests/golden/atlas_assets.npz), its start camera (player (256, 22, 256)
+    + 3 up, identity rotation, fov pi/2), the G-buffer size
+    GBuffer.init(1280, 720) (game.zig:91), the shaders' own step caps (192
+    primary, primary.comp.glsl:43; 48 shadow, secondary.comp.glsl:41), its
+    rgba8 G-buffer images (gbuffer.zig:10-13: the shadow origin uses the
+    stored normal) and SUN_DIR (camera.glsl:8).  A step is one vt_render
     with the world resident in HBM."""
     import math
 
@@ -595,8 +628,8 @@ def bench_engine(args):
     cam.position[:] = [256.0, 25.0, 256.0, 0.0]
     cam.view[:] = [1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0]
     cam.fov = float(np.float32(math.pi / 2.0))
-    W, H, sun = 1920, 1080, (0.4, 0.8, 0.3)
-    prm = vt.params(W, H, 192, 48, sun)
+    W, H, sun = 1280, 720, SUN_DIR
+    prm = vt.params(W, H, 192, 48, sun, vt.VT_FLAG_GBUF_UNORM8)
     fs = vt.Frame.alloc(W, H, stats=True)
     vt.render(ctx, world, cam, fs, prm)
     torch.cuda.synchronize()
@@ -614,17 +647,21 @@ def bench_engine(args):
         ow = O.BrickmapWorld(512, chunks, pool, atlas)
         threads = min(len(os.sched_getaffinity(0)), 16)
         t0 = time.perf_counter()
-        o = O.render(ow, cam, W, H, 192, 48, sun, threads=threads, gbuffer=False)
+        o = O.render(ow, cam, W, H, 192, 48, sun, flags=vt.VT_FLAG_GBUF_UNORM8,
+                     threads=threads, gbuffer=False)
         dt = time.perf_counter() - t0
         same = bool(np.array_equal(f.hits.cpu().numpy().view(np.uint32), o["hits"]))
         cpu = {"value": round((W * H + int(o["stats"][2])) / dt / 1e6, 3), "unit": "Mrays/s",
                "cores": threads, "kind": "port",
                "sample": f"the same full frame, {threads} threads, {dt:.2f} s wall"}
-    abytes = 4 * (st["l1_fetches"] + st["l0_fetches"]) + OUT_BYTES_PER_PIXEL * W * H
+    # lower bound of the timed kernel's reads (see bench_brickmap)
+    abytes = LOWER_BYTES_PER_STEP * (st["primary_steps"] + st["shadow_steps"]) + \
+        OUT_BYTES_PER_PIXEL * W * H
+    literal_bytes = 4 * (st["l1_fetches"] + st["l0_fetches"]) + OUT_BYTES_PER_PIXEL * W * H
     achieved = abytes / (ms * 1e-3) / 1e9
     line = {
-        "metric": "Mrays/s primary+shadow, the engine's own 1080p frame (Game.init world and "
-                  "camera, shader step caps 192/48)",
+        "metric": "Mrays/s primary+shadow, the engine's own frame (Game.init world and camera, "
+                  "1280x720 G-buffer, shader step caps 192/48, rgba8 G-buffer, SUN_DIR)",
         "value": round(rays / (ms * 1e-3) / 1e6, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -638,13 +675,15 @@ def bench_engine(args):
         "data": "the reference's generator and .vox models",
         "config": {
             "workload": "procgen(512, 0, 0) world in the chunk/pool layout + 29-slot model "
-                        "atlas; start camera; 1920x1080; caps 192/48",
+                        "atlas; start camera; 1280x720; caps 192/48; rgba8 G-buffer; SUN_DIR",
             "primary_hits": st["primary_hits"],
             "dda_steps": st["primary_steps"] + st["shadow_steps"],
             "hits_equal_oracle": same,
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "bytes": "lower bound: 1 B per step + 28 B per pixel",
+                     "literal_loop_gbs": round(literal_bytes / (ms * 1e-3) / 1e9, 2)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -824,6 +863,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": measured_traffic(f"k_render_dense/{args.traversal}/512cube_1080p"),
+            "issue": measured_issue(f"k_render_dense/{args.traversal}/512cube_1080p", ms),
         },
         "cpu_baseline": cpu,
         "parity": parity,
