@@ -48,6 +48,22 @@ def algorithmic_bytes(stats, traversal, npix):
     return b + OUT_BYTES_PER_PIXEL * npix
 
 
+def counted_stats(vt, torch, ctx, world, jobs, free):
+    """Statistics summed over views, jobs = [(camera, W, H, params maker)]
+    where params maker(flags) returns the view's params.  free=False:
+    traceMap's steps (the statistics kernel variant); free=True: the steps
+    the free-region kernel executes (VT_FLAG_FREE_MAP_ON: a ray ends where
+    nothing is left to hit), the work the timed kernel does when it takes
+    the test."""
+    stats = torch.zeros(8, dtype=torch.int64, device="cuda")
+    for cam, W, H, mk in jobs:
+        f = vt.Frame.alloc(W, H, shaded=False)
+        f.stats = stats
+        vt.render(ctx, world, cam, f, mk(vt.VT_FLAG_FREE_MAP_ON if free else 0))
+    torch.cuda.synchronize()
+    return dict(zip(vt._native.STAT_NAMES, stats.cpu().tolist()))
+
+
 def cpu_baseline(grid, pal, cam, W, H, max_steps, sun):
     """The oracle (CPU restatement) on this host's cores, one whole frame.
     Returns (baseline record, the oracle's outputs) -- the outputs serve as
@@ -124,6 +140,13 @@ def bench_batched(args):
     st = dict(zip(vt._native.STAT_NAMES, stats.cpu().tolist()))
     counted.close()
     rays = npix + st["primary_hits"]
+    # the batch takes the free-region test as vt_render does for these frames
+    free = vt.free_map_used(world, frames[0], vt.params(views[0][1], views[0][2], ms_cap, ms_cap))
+    ex = st
+    if free:
+        ex = counted_stats(vt, torch, ctx, world,
+                           [(c, W, H, lambda fl, W=W, H=H: vt.params(W, H, ms_cap, ms_cap, S.SUN, fl))
+                            for c, W, H in views], True)
 
     print("[bench] plain: one vt_render per view", file=sys.stderr, flush=True)
     vprm = [vt.params(W, H, ms_cap, ms_cap) for _, W, H in views]
@@ -140,7 +163,9 @@ def bench_batched(args):
     batch.capture_graph()
     print(f"[bench] graph replays x{args.steps}", file=sys.stderr, flush=True)
     ms = timed(torch, stream, batch.graph_launch, args.steps, args.warmup)
-    abytes = st["primary_steps"] + st["shadow_steps"] + OUT_BYTES_PER_PIXEL * npix
+    if TRAVERSALS[best.traversal] != "flat":
+        ex = st  # the bitmask traversals run every ray to its end
+    abytes = ex["primary_steps"] + ex["shadow_steps"] + OUT_BYTES_PER_PIXEL * npix
     achieved = abytes / (ms * 1e-3) / 1e9
     line = {
         "metric": "Mrays/s primary+shadow, 64 uneven cameras on a 1024³ grid, one persistent "
@@ -162,6 +187,8 @@ def bench_batched(args):
             "pixels": npix,
             "shadow_rays": st["primary_hits"],
             "dda_steps": st["primary_steps"] + st["shadow_steps"],
+            "free_region_test": free and TRAVERSALS[best.traversal] == "flat",
+            "executed_steps": ex["primary_steps"] + ex["shadow_steps"],
             "tuning": {"tile": {3: "8x8", 4: "16x4", 5: "32x2"}[best.tile],
                        "chunk": best.chunk,
                        "traversal": TRAVERSALS[best.traversal],
@@ -218,13 +245,22 @@ def bench_brickmap(args):
                      args.warmup)
     npix = W * H
     rays = npix + st["primary_hits"]
-    # what the timed kernel must read at least: one byte per traceMap step
+    # the steps the timed kernel executes: with the free-region test (the
+    # default here) rays end where nothing is left to hit; the block-level
+    # steps are those of the dense world with the same voxels (a solid model
+    # per block: the literal loop ends at the hit block), whose statistics
+    # count them
+    free = vt.free_map_used(world, f, prm)
+    ex = st
+    if free:
+        ex = counted_stats(vt, torch, ctx, dense,
+                           [(cam, W, H, lambda fl: vt.params(W, H, cap, cap, S.SUN, fl))], True)
+    # what the timed kernel must read at least: one byte per executed step
     # (an occupancy byte in the empty-space loop, a 4-byte chunk word in the
     # literal loop) + the outputs; the literal loop's own reads -- 4 B chunk,
     # 4 B pool, 4 B atlas per step -- are the upper figure
-    abytes = LOWER_BYTES_PER_STEP * (st["primary_steps"] + st["shadow_steps"]) + \
+    abytes = LOWER_BYTES_PER_STEP * (ex["primary_steps"] + ex["shadow_steps"]) + \
         OUT_BYTES_PER_PIXEL * npix
-    literal_bytes = 4 * (st["l1_fetches"] + st["l0_fetches"]) + OUT_BYTES_PER_PIXEL * npix
     achieved = abytes / (ms * 1e-3) / 1e9
     line = {
         "metric": "Mrays/s primary+shadow, 1080p/512³ grid in the engine's chunk/pool/atlas "
@@ -248,11 +284,12 @@ def bench_brickmap(args):
             "chunk_reads": st["l1_fetches"],
             "pool_atlas_reads": st["l0_fetches"],
             "dda_steps": st["primary_steps"] + st["shadow_steps"],
+            "free_region_test": free,
+            "executed_steps": ex["primary_steps"] + ex["shadow_steps"],
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "bytes": "lower bound: 1 B per step + 28 B per pixel",
-                     "literal_loop_gbs": round(literal_bytes / (ms * 1e-3) / 1e9, 2)},
+                     "bytes": "lower bound: 1 B per executed step + 28 B per pixel"},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -353,6 +390,16 @@ def measure_diff(args, scene):
     torch.cuda.synchronize()
     st = dict(zip(vt._native.STAT_NAMES, stats.cpu().tolist()))
     nhit = st["primary_hits"]
+    # the forward kernels take the free-region test as vt_render does for
+    # these frames (the batch kernel follows the same rule)
+    f0 = frames_without("target")[0]
+    free = vt.free_map_used(world, f0, prm(f0.width, f0.height, 0, None))
+    ex = st
+    if free:
+        ex = counted_stats(vt, torch, ctx, world,
+                           [(c, f.width, f.height,
+                             lambda fl, f=f: prm(f.width, f.height, fl, None))
+                            for c, f in zip(cams, frames)], True)
 
     def zero():
         grad.zero_()
@@ -442,7 +489,7 @@ def measure_diff(args, scene):
     rays = npix + nhit
     # backward: 12 B record + 16 B target per pixel, 12 B albedo + 12 B gradient per hit
     bwd_bytes = 28 * npix + 24 * nhit
-    fwd_bytes = (st["primary_steps"] + st["shadow_steps"] + OUT_BYTES_PER_PIXEL * npix +
+    fwd_bytes = (ex["primary_steps"] + ex["shadow_steps"] + OUT_BYTES_PER_PIXEL * npix +
                  12 * nhit)
     fwd_gbs = fwd_bytes / (lambert_ms * 1e-3) / 1e9
     line = {
@@ -477,6 +524,9 @@ def measure_diff(args, scene):
             "torch_peak_bytes": int(torch.cuda.max_memory_allocated()),
             "backward_gbs": round(bwd_bytes / (bwd_ms * 1e-3) / 1e9, 2),
             "rays": rays,
+            "dda_steps": st["primary_steps"] + st["shadow_steps"],
+            "free_region_test": free,
+            "executed_steps": ex["primary_steps"] + ex["shadow_steps"],
         },
         "roofline": {"bound": "hbm", "achieved": round(fwd_gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(fwd_gbs / HBM_PEAK_GBS, 5), "traffic": None},
@@ -791,15 +841,15 @@ def main():
     world = vt.World.dense(ctx, grid, pal)
     trav = TRAVERSALS.index(args.traversal)
 
-    # one counted launch (statistics kernel variant) for rays and bytes
-    fs = vt.Frame.alloc(W, H, stats=True)
-    vt.render(ctx, world, cam, fs, vt.params(W, H, max_steps, max_steps, S.SUN, 0, trav))
-    torch.cuda.synchronize()
-    stats = fs.stats_dict()
-    del fs
-
     f = vt.Frame.alloc(W, H)
     prm = vt.params(W, H, max_steps, max_steps, S.SUN, 0, trav)
+    # counted launches for rays and bytes: traceMap's steps (statistics
+    # kernel variant) and, when the timed launch takes the free-region test
+    # (the default for this world and these caps), the steps it executes
+    free = vt.free_map_used(world, f, prm)
+    job = [(cam, W, H, lambda fl: vt.params(W, H, max_steps, max_steps, S.SUN, fl, trav))]
+    stats = counted_stats(vt, torch, ctx, world, job, False)
+    ex = counted_stats(vt, torch, ctx, world, job, True) if free else stats
     stream = torch.cuda.current_stream()
     print(f"[bench] warmup {args.warmup}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
@@ -822,9 +872,11 @@ def main():
     npix = W * H
     rays = npix + stats["primary_hits"]
     value = rays / (ms * 1e-3) / 1e6
-    abytes = algorithmic_bytes(stats, args.traversal, npix)
+    abytes = algorithmic_bytes(ex, args.traversal, npix)
     achieved = abytes / (ms * 1e-3) / 1e9
     steps_total = stats["primary_steps"] + stats["shadow_steps"]
+    executed = ex["primary_steps"] + ex["shadow_steps"]
+    key = "flat_free" if free else args.traversal
 
     cpu = parity = None
     if not args.no_cpu:
@@ -855,6 +907,8 @@ def main():
             "shadow_rays": stats["primary_hits"],
             "dda_steps": steps_total,
             "gsteps_per_s": round(steps_total / (ms * 1e-3) / 1e9, 2),
+            "free_region_test": free,
+            "executed_steps": executed,
         },
         "roofline": {
             "bound": "hbm",
@@ -862,8 +916,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": measured_traffic(f"k_render_dense/{args.traversal}/512cube_1080p"),
-            "issue": measured_issue(f"k_render_dense/{args.traversal}/512cube_1080p", ms),
+            "traffic": measured_traffic(f"k_render_dense/{key}/512cube_1080p"),
+            "issue": measured_issue(f"k_render_dense/{key}/512cube_1080p", ms),
         },
         "cpu_baseline": cpu,
         "parity": parity,

@@ -84,6 +84,23 @@ typedef struct vt_camera {
                                       /* vignette of blit.fragment.glsl:32-35 */
                                       /* (not with LAMBERT); frame.display,   */
                                       /* when set, gets it as RGBA8           */
+/* Free-region test (flat traversal of dense worlds; brickmap worlds with a
+ * power-of-two dim).  The library keeps, per world, a bitmap of the 16^3
+ * regions from which no ray of a given direction octant can meet a nonempty
+ * block (every region ahead of it on all three axes is empty: traceMap's
+ * blocks only move in the travel direction), and ends a ray as a miss when
+ * it stands in such a region.  Every output is unchanged except the step
+ * counts, so the test is never made for a frame that stores them
+ * (frame.steps).  Statistics (frame.stats) count traceMap's steps; with
+ * VT_FLAG_FREE_MAP_ON on a dense world they count the steps executed (rays
+ * ending early).  Default: made when a step cap is at least VT_FREE_MIN_CAP
+ * and some octant has at least 1/VT_FREE_MIN_SHARE of the world's regions
+ * free (below either, the test costs more than it saves); these flags force
+ * it either way (not both).                                                */
+#define VT_FLAG_FREE_MAP_ON (1u << 5)
+#define VT_FLAG_FREE_MAP_OFF (1u << 6)
+#define VT_FREE_MIN_CAP 512
+#define VT_FREE_MIN_SHARE 16
 
 /* traversal storage variants for dense worlds (identical hits by contract) */
 #define VT_TRAVERSAL_FLAT 0u     /* one uint8 gather per executed step       */
@@ -282,6 +299,14 @@ int vt_world_info(const vt_world* world, uint32_t* dim, size_t* device_bytes);
  * dispatches of Game.render (game.zig:241-248).                           */
 int vt_render(vt_ctx* ctx, const vt_world* world, const vt_camera* camera,
               const vt_render_params* params, const vt_frame* frame, void* stream);
+
+/* Whether vt_render with these arguments makes the free-region test
+ * (VT_FLAG_FREE_MAP_ON above): *used = 1 when its rays end early in free
+ * regions, 0 when every ray runs to its end.  No counterpart in the
+ * reference, whose traceMap runs every ray to its end; for callers that
+ * need to know whether a frame's statistics are traceMap's step counts.   */
+int vt_free_map_used(const vt_world* world, const vt_render_params* params,
+                     const vt_frame* frame, int* used);
 
 /* ---- present pass ---------------------------------------------------------- */
 /* The blit pass after the composite (blit.fragment.glsl:32-35), as a

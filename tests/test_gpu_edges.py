@@ -6,8 +6,7 @@ away (all sky), exactly on lattice points, inside a solid voxel, looking
 exactly along each axis (zero direction components: the 0.001
 substitution), grazing a face; images 1x1, 1xN, Nx1 and ragged tile edges;
 step caps 0, 1, 2 and a zero shadow cap; both traversals and the reference
-layout of the same worlds; frames large enough for the packed long shadow
-rays.  The bar is the one of the main parity tests:
+layout of the same worlds; frames that fill every CU's block slots.  The bar is the one of the main parity tests:
 hit records, step counts, G-buffer position/normal/illumination and the
 statistics counters bit-exact, colours within 1e-5.
 """
@@ -167,14 +166,12 @@ def test_edge_diagonal_cameras(ctx):
     w.close()
 
 
-def test_edge_packed_shadow_rays(ctx):
+def test_edge_large_frames(ctx):
     """Frames with enough blocks (16x16 pixels each) to fill every CU's 8
-    block slots, for which the kernels pack each block's long shadow rays
-    after a first segment of one empty-space iteration and finish them 64
-    to a wave (shadow caps of at least 256 steps; the smaller caps here take
-    the unpacked path and the segment's end): tied suns (two- and three-axis
-    ties every step), suns of either sign, lattice-point cameras, on the
-    dense and the reference layout, against the oracle."""
+    block slots, shadow caps around the empty-space iteration's length and
+    long ones: tied suns (two- and three-axis ties every step), suns of
+    either sign, lattice-point cameras, on the dense and the reference
+    layout, against the oracle."""
     W, H = 1280, 720
     cus = ctx.info()["compute_units"]
     assert ((W + 15) // 16) * ((H + 15) // 16) >= 8 * cus
@@ -194,17 +191,18 @@ def test_edge_packed_shadow_rays(ctx):
         for cname, cam in (cameras(n)[1], cameras(n)[7]):
             g = gpu_render(ctx, w, cam, W, H, 8 * n, cap, sun=sun)
             o = O.render(ow, cam, W, H, 8 * n, cap, sun)
-            compare(g, o, label=f"packed {wname} {cname} sun={sun} shadow cap {cap}")
+            compare(g, o, label=f"large {wname} {cname} sun={sun} shadow cap {cap}")
             # the reference layout: sub-voxel steps inside the hit block first
             g = gpu_render(ctx, wb, cam, W, H, 64 * n, 8 * cap, sun=sun)
             o = O.render(owb, cam, W, H, 64 * n, 8 * cap, sun)
-            compare(g, o, stats=False, label=f"packed brickmap {wname} {cname} sun={sun}")
+            compare(g, o, stats=False, label=f"large brickmap {wname} {cname} sun={sun}")
         w.close()
         wb.close()
 
 
-def test_edge_packed_present_fused(ctx):
-    """The fused present pass on the packed path (1280x720, shadow cap 512):
+def test_edge_large_present_fused(ctx):
+    """The fused present pass on a frame that fills the GPU (1280x720,
+    shadow cap 512):
     shaded and display images equal to a plain render followed by
     vt_present, on the dense and the reference layout."""
     import torch

@@ -1,0 +1,226 @@
+"""The free-region test on the GPU: a ray standing in a 16^3 region from
+which every region ahead of it (on all three axes, in its travel direction)
+is empty ends as a miss.  Every output but the step counts must equal the
+oracle's, which runs every ray to its end.
+
+Worlds with large empty parts (a few boxes and single voxels in a larger
+grid, at 16^3 .. 256^3: 1 to 16 regions per axis), cameras inside and
+outside the grid looking into all eight direction octants, suns of either
+sign; the map kept current by edits on both layouts (a stale map would end
+rays before voxels added after it was built); the flag rules.
+"""
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+import vt
+from test_gpu_parity import compare, poison
+from vt import camera as vcam
+from vt import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+ON = vt.VT_FLAG_FREE_MAP_ON
+OFF = vt.VT_FLAG_FREE_MAP_OFF
+
+
+def log(msg):
+    print(f"[freemap] {msg}", flush=True)
+
+
+def boxes_world(n, seed, n_boxes=6, n_single=12):
+    """A dim n grid holding a few boxes and single voxels: most 16^3 regions
+    empty, and empty orthants in every octant from many of them."""
+    rng = np.random.default_rng(seed)
+    g = np.zeros((n, n, n), np.uint8)
+    for _ in range(n_boxes):
+        s = rng.integers(1, max(2, n // 6), 3)
+        o = rng.integers(0, n - s + 1)
+        g[o[2]:o[2] + s[2], o[1]:o[1] + s[1], o[0]:o[0] + s[0]] = rng.integers(1, 9)
+    for _ in range(n_single):
+        x, y, z = rng.integers(0, n, 3)
+        g[z, y, x] = rng.integers(1, 9)
+    return g
+
+
+def render_free(ctx, world, cam, W, H, cap_p, cap_s, sun, flags=ON, gbuffer=True):
+    import torch
+    f = poison(vt.Frame.alloc(W, H, gbuffer=gbuffer))
+    vt.render(ctx, world, cam, f, vt.params(W, H, cap_p, cap_s, sun, flags))
+    torch.cuda.synchronize()
+    out = {"shaded": f.shaded.cpu().numpy(), "hits": f.hits.cpu().numpy().view(np.uint32)}
+    if gbuffer:
+        out.update(color=f.color.cpu().numpy(), normal=f.normal.cpu().numpy(),
+                   position=f.position.cpu().numpy(),
+                   illumination=f.illumination.cpu().numpy())
+    return out
+
+
+def random_cameras(n, rng, count):
+    """Eyes inside and outside the grid, targets anywhere: every octant."""
+    out = []
+    for _ in range(count):
+        inside = rng.random() < 0.5
+        lo, hi = (0.0, float(n)) if inside else (-0.5 * n, 1.5 * n)
+        eye = rng.uniform(lo, hi, 3)
+        tgt = rng.uniform(-0.5 * n, 1.5 * n, 3)
+        if np.linalg.norm(tgt - eye) < 1.0:
+            tgt = eye + 7.0
+        out.append(vcam.look_at(tuple(eye), tuple(tgt), float(rng.uniform(40, 100))))
+    return out
+
+
+SUNS = [S.SUN, (-0.5, 0.7, -0.2), (0.3, -0.8, 0.4), (-0.6, -0.5, -0.6)]
+
+
+@pytest.mark.parametrize("n", [16, 32, 64, 256])
+def test_random_boxes_every_octant(ctx, n):
+    import brickmap_util as BU
+    pal = S.palette()
+    rng = np.random.default_rng(1000 + n)
+    grid = boxes_world(n, 7 + n)
+    wd = vt.World.dense(ctx, grid, pal)
+    dim, chunks, pool, atlas = BU.embedding_a(grid, pal)
+    wb = vt.World.brickmap(ctx, dim, chunks, pool, atlas)
+    ow = O.DenseWorld(grid, pal)
+    W, H = 48, 40
+    cap = 4 * n
+    for k, cam in enumerate(random_cameras(n, rng, 12)):
+        sun = SUNS[k % len(SUNS)]
+        o = O.render(ow, cam, W, H, cap, cap, sun)
+        label = f"boxes n={n} camera {k} sun={sun}"
+        log(label)
+        compare(render_free(ctx, wd, cam, W, H, cap, cap, sun), o, stats=False,
+                label=label + " dense")
+        gb = render_free(ctx, wb, cam, W, H, cap, cap, sun, gbuffer=False)
+        assert np.array_equal(gb["hits"], o["hits"]), f"{label} reference layout: hits"
+        assert np.max(np.abs(gb["shaded"] - o["shaded"])) <= 1e-5, f"{label} reference layout"
+    wd.close()
+    wb.close()
+
+
+def test_map_follows_dense_edits(ctx):
+    """A box added to an empty world after the map was built is hit; removed
+    again, every ray misses."""
+    pal = S.palette()
+    n = 64
+    grid = np.zeros((n, n, n), np.uint8)
+    w = vt.World.dense(ctx, grid, pal)
+    cam = vcam.look_at((-20.0, 70.0, -25.0), (32.0, 30.0, 32.0), 60.0)
+    W, H = 64, 48
+    o = O.render(O.DenseWorld(grid, pal), cam, W, H, 512, 512, S.SUN)
+    compare(render_free(ctx, w, cam, W, H, 512, 512, S.SUN), o, stats=False, label="empty")
+    assert np.all(o["hits"][..., 0] == 0xFFFFFFFF)
+    z, y, x = np.mgrid[20:44, 10:30, 24:40]
+    xyz = np.stack([x.ravel(), y.ravel(), z.ravel()], 1).astype(np.uint32)
+    w.set_voxels(xyz, np.full(len(xyz), 5, np.uint32))
+    grid[20:44, 10:30, 24:40] = 5
+    o = O.render(O.DenseWorld(grid, pal), cam, W, H, 512, 512, S.SUN)
+    assert np.mean(o["hits"][..., 0] != 0xFFFFFFFF) > 0.05
+    compare(render_free(ctx, w, cam, W, H, 512, 512, S.SUN), o, stats=False, label="box added")
+    w.set_voxels(xyz, np.zeros(len(xyz), np.uint32))
+    grid[:] = 0
+    o = O.render(O.DenseWorld(grid, pal), cam, W, H, 512, 512, S.SUN)
+    compare(render_free(ctx, w, cam, W, H, 512, 512, S.SUN), o, stats=False, label="box removed")
+    w.close()
+
+
+def test_map_follows_brickmap_edits(ctx):
+    """An empty reference-layout world built up by an engine-style edit stream
+    (the 29 model slots of the engine's assets) in three batches: after each,
+    the test on gives exactly the images of the test off."""
+    from test_gpu_edit import atlas_fixture, engine_edit_stream
+    n = 64
+    w = vt.World.empty_brickmap(ctx, n, atlas_fixture())
+    cam = vcam.look_at((-20.0, 40.0, -25.0), (32.0, 6.0, 32.0), 60.0)
+    W, H = 64, 48
+    before = render_free(ctx, w, cam, W, H, 1536, 512, S.SUN, gbuffer=False)
+    assert np.all(before["hits"][..., 0] == 0xFFFFFFFF)
+    ed = engine_edit_stream(n, 9)
+    for k, part in enumerate(np.array_split(ed, 3)):
+        w.set_voxels(part[:, :3], part[:, 3])
+        on = render_free(ctx, w, cam, W, H, 1536, 512, S.SUN, gbuffer=False)
+        off = render_free(ctx, w, cam, W, H, 1536, 512, S.SUN, flags=OFF, gbuffer=False)
+        log(f"brickmap edit batch {k}: {np.mean(off['hits'][..., 0] != 0xFFFFFFFF):.2f} hit")
+        assert np.array_equal(on["hits"], off["hits"]), f"batch {k}: hits"
+        assert np.array_equal(on["shaded"].view(np.uint32), off["shaded"].view(np.uint32))
+    assert np.mean(off["hits"][..., 0] != 0xFFFFFFFF) > 0.3
+    w.close()
+
+
+def test_on_off_auto_agree_c1_row_band(ctx):
+    """The headline world: forced on, forced off and the default give the
+    same images (the default takes the test: caps of 2048)."""
+    import torch
+    grid = S.generate(S.spec("c1"))
+    pal = S.palette()
+    w = vt.World.dense(ctx, grid, pal)
+    del grid
+    cam, W, H = S.views("c1")[0]
+    imgs = []
+    for flags in (0, ON, OFF):
+        f = vt.Frame.alloc(W, H)
+        vt.render(ctx, w, cam, f, vt.params(W, H, 2048, 2048, S.SUN, flags))
+        torch.cuda.synchronize()
+        imgs.append((f.hits.cpu().numpy().view(np.uint32), f.shaded.cpu().numpy().view(np.uint32)))
+    for k in (1, 2):
+        assert np.array_equal(imgs[0][0], imgs[k][0])
+        assert np.array_equal(imgs[0][1], imgs[k][1])
+    # which launches take the test (vt_free_map_used)
+    p = vt.params(W, H, 2048, 2048, S.SUN)
+    assert vt.free_map_used(w, vt.Frame.alloc(W, H), p)
+    assert not vt.free_map_used(w, vt.Frame.alloc(W, H, steps=True), p)
+    assert not vt.free_map_used(w, vt.Frame.alloc(W, H, stats=True), p)
+    assert vt.free_map_used(w, vt.Frame.alloc(W, H, stats=True),
+                            vt.params(W, H, 2048, 2048, S.SUN, ON))
+    assert not vt.free_map_used(w, vt.Frame.alloc(W, H), vt.params(W, H, 2048, 2048, S.SUN, OFF))
+    assert not vt.free_map_used(w, vt.Frame.alloc(W, H), vt.params(W, H, 192, 48, S.SUN))
+    assert not vt.free_map_used(w, vt.Frame.alloc(W, H),
+                                vt.params(W, H, 2048, 2048, S.SUN, 0, vt.VT_TRAVERSAL_BITMASK))
+    w.close()
+    # a world with free regions in no octant: the default leaves the test off
+    g = boxes_world(64, 3)
+    g[:, :, :] |= (np.arange(64)[None, None, :] % 16 == 15).astype(np.uint8)  # a wall per region
+    w = vt.World.dense(ctx, g, pal)
+    assert not vt.free_map_used(w, vt.Frame.alloc(64, 64), vt.params(64, 64, 2048, 2048, S.SUN))
+    w.close()
+
+
+def test_statistics_of_executed_steps(ctx):
+    """Statistics with the test forced on count the steps the kernel
+    executed: fewer than traceMap's (rays end early), with the same hits,
+    occluded shadow rays, material reads and entity hits; the images equal."""
+    import torch
+    n = 256
+    grid = boxes_world(n, 99, n_boxes=10, n_single=40)
+    w = vt.World.dense(ctx, grid, S.palette())
+    cam = vcam.look_at((-40.0, 180.0, -60.0), (128.0, 40.0, 128.0), 70.0)
+    W, H = 320, 240
+    out = {}
+    for name, flags in (("ref", 0), ("on", ON)):
+        f = poison(vt.Frame.alloc(W, H, stats=True))
+        vt.render(ctx, w, cam, f, vt.params(W, H, 1024, 1024, S.SUN, flags))
+        torch.cuda.synchronize()
+        out[name] = (f.stats_dict(), f.hits.cpu().numpy().view(np.uint32),
+                     f.shaded.cpu().numpy().view(np.uint32))
+    ref, on = out["ref"][0], out["on"][0]
+    log(f"steps executed {on['primary_steps']} + {on['shadow_steps']} of traceMap's "
+        f"{ref['primary_steps']} + {ref['shadow_steps']}")
+    for k in ("primary_hits", "occluded"):
+        assert on[k] == ref[k], k
+    assert on["primary_steps"] < ref["primary_steps"]
+    assert on["shadow_steps"] <= ref["shadow_steps"]
+    assert np.array_equal(out["ref"][1], out["on"][1])
+    assert np.array_equal(out["ref"][2], out["on"][2])
+    w.close()
+
+
+def test_flag_rules(ctx):
+    grid = np.zeros((16, 16, 16), np.uint8)
+    w = vt.World.dense(ctx, grid, S.palette())
+    f = vt.Frame.alloc(8, 8)
+    cam = vcam.look_at((8.0, 8.0, -10.0), (8.0, 8.0, 8.0), 60.0)
+    with pytest.raises(vt.VtError) as e:
+        vt.render(ctx, w, cam, f, vt.params(8, 8, 64, 64, S.SUN, ON | OFF))
+    assert e.value.status == vt._native.VT_ERR_INVALID_ARGUMENT
+    w.close()

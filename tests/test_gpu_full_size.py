@@ -15,7 +15,7 @@ import pytest
 
 import oracle_ffi as O
 import vt
-from test_gpu_parity import compare
+from test_gpu_parity import POISON, compare, poison
 from vt import scenes as S
 
 pytestmark = pytest.mark.gpu
@@ -32,8 +32,8 @@ def c2(ctx):
 
 
 def _render(ctx, world, cam, W, H, flags=0, traversal=vt.VT_TRAVERSAL_FLAT, stats=False,
-            gbuffer=False, **kw):
-    f = vt.Frame.alloc(W, H, gbuffer=gbuffer, steps=True, stats=stats)
+            gbuffer=False, steps=True, **kw):
+    f = poison(vt.Frame.alloc(W, H, gbuffer=gbuffer, steps=steps, stats=stats))
     vt.render(ctx, world, cam, f, vt.params(W, H, S.MAX_STEPS["c2"], S.MAX_STEPS["c2"], S.SUN,
                                             flags, traversal, **kw))
     return f
@@ -47,7 +47,7 @@ def _dict(f):
         "normal": f.normal.cpu().numpy(),
         "position": f.position.cpu().numpy(),
         "illumination": f.illumination.cpu().numpy(),
-        "steps": f.steps.cpu().numpy().view(np.uint32),
+        **({} if f.steps is None else {"steps": f.steps.cpu().numpy().view(np.uint32)}),
     }
 
 
@@ -60,6 +60,8 @@ def test_c2_views_vs_oracle_bands(ctx, c2):
         cam, W, H = views[i]
         g = _dict(_render(ctx, world, cam, W, H, stats=True, gbuffer=True))
         g["fast"] = _dict(_render(ctx, world, cam, W, H, gbuffer=True))
+        # no step counts: the free-region test ends rays early
+        g["free"] = _dict(_render(ctx, world, cam, W, H, gbuffer=True, steps=False))
         torch.cuda.synchronize()
         y0 = (H // 2) & ~7
         rows = (y0, min(H, y0 + 8))
@@ -74,7 +76,7 @@ def test_c2_batch_and_traversals_equal_plain(ctx, c2):
     plain = []
     for cam, W, H in views:
         plain.append(_render(ctx, world, cam, W, H))
-    frames = [vt.Frame.alloc(W, H, steps=True) for _, W, H in views]
+    frames = [poison(vt.Frame.alloc(W, H, steps=True)) for _, W, H in views]
     prm = vt.params(0, 0, S.MAX_STEPS["c2"], S.MAX_STEPS["c2"], S.SUN)
     print("[full-size] c2 batch of 64 views", flush=True)
     b = vt.Batch(ctx, world, [(cam, f) for (cam, _, _), f in zip(views, frames)], prm,
@@ -86,6 +88,23 @@ def test_c2_batch_and_traversals_equal_plain(ctx, c2):
             a = getattr(f, k).cpu().numpy().view(np.uint32)
             c = getattr(r, k).cpu().numpy().view(np.uint32)
             assert np.array_equal(a, c), f"c2 view {i}: batch {k} differs from vt_render"
+    b.close()
+    # frames without step counts: the batch (and vt_render) end rays early in
+    # free regions; every other output is unchanged
+    print("[full-size] c2 batch of 64 views, free-region test", flush=True)
+    frames = [poison(vt.Frame.alloc(W, H)) for _, W, H in views]
+    b = vt.Batch(ctx, world, [(cam, f) for (cam, _, _), f in zip(views, frames)], prm,
+                 vt.tuning(chunk=2))
+    b.launch()
+    torch.cuda.synchronize()
+    for i, ((cam, W, H), f, r) in enumerate(zip(views, frames, plain)):
+        rf = _render(ctx, world, cam, W, H, steps=False)
+        torch.cuda.synchronize()
+        for k in ("hits", "shaded"):
+            c = getattr(r, k).cpu().numpy().view(np.uint32)
+            for src, g in (("batch", f), ("vt_render", rf)):
+                a = getattr(g, k).cpu().numpy().view(np.uint32)
+                assert np.array_equal(a, c), f"c2 view {i}: {src} {k} with the free-region test"
     b.close()
     print("[full-size] c2 flat vs bitmask (4^3 and 8^3 bricks) on every view", flush=True)
     for i, ((cam, W, H), r) in enumerate(zip(views, plain)):
@@ -113,15 +132,28 @@ def test_c4_lambert_batch_compact_albedo(ctx, c2):
     views = S.views("c4")
     prm = vt.params(0, 0, S.MAX_STEPS["c4"], S.MAX_STEPS["c4"], S.SUN, flags, 0, 0.25,
                     albedo.view(-1))
-    frames = [vt.Frame.alloc(W, H, steps=True) for _, W, H in views]
+    frames = [poison(vt.Frame.alloc(W, H, steps=True)) for _, W, H in views]
     print("[full-size] c4 Lambert batch, compact albedo", flush=True)
     b = vt.Batch(ctx, world, [(cam, f) for (cam, _, _), f in zip(views, frames)], prm,
                  vt.tuning(chunk=2))
     b.launch()
     torch.cuda.synchronize()
+    # the same batch without step counts (free-region test on): same images
+    frames_free = [poison(vt.Frame.alloc(W, H)) for _, W, H in views]
+    bf = vt.Batch(ctx, world, [(cam, f) for (cam, _, _), f in zip(views, frames_free)], prm,
+                  vt.tuning(chunk=2))
+    bf.launch()
+    torch.cuda.synchronize()
+    for i, (f, ff) in enumerate(zip(frames, frames_free)):
+        for k in ("hits", "shaded"):
+            a = getattr(ff, k).cpu().numpy().view(np.uint32)
+            c = getattr(f, k).cpu().numpy().view(np.uint32)
+            assert np.array_equal(a, c), f"c4 view {i}: {k} with the free-region test"
+    bf.close()
+    del frames_free
     for i in (0, 23, 47):
         cam, W, H = views[i]
-        r = vt.Frame.alloc(W, H, steps=True)
+        r = poison(vt.Frame.alloc(W, H, steps=True))
         vt.render(ctx, world, cam, r, vt.params(W, H, S.MAX_STEPS["c4"], S.MAX_STEPS["c4"],
                                                  S.SUN, flags, 0, 0.25, albedo.view(-1)))
         torch.cuda.synchronize()
@@ -155,6 +187,31 @@ def test_c4_lambert_batch_compact_albedo(ctx, c2):
         ucode, first = np.unique(code, return_index=True)
         assert np.all(np.diff(rank[first]) > 0), f"c4 view {i}: compact index not increasing"
     b.close()
+
+
+def test_c2_repeated_renders_identical(ctx, c2):
+    """The same large views rendered again and again, into frames filled
+    with POISON first: every render writes every pixel and gives the same
+    records, with step counts stored (free-region test off) and without
+    (test on).  A kernel whose lanes exchange work through LDS once lost or
+    mixed up some pixels' records from one render to the next here."""
+    import torch
+    grid, pal, world = c2
+    views = S.views("c2")
+    for i in (2, 35):
+        cam, W, H = views[i]
+        for steps in (True, False):
+            ref = None
+            for k in range(6):
+                f = _render(ctx, world, cam, W, H, steps=steps)
+                torch.cuda.synchronize()
+                h = f.hits.cpu().numpy().view(np.uint32)
+                assert not np.any(h == POISON), f"view {i} render {k}: pixels left unwritten"
+                if ref is None:
+                    ref = h.copy()
+                else:
+                    n = int(np.sum(np.any(h != ref, -1)))
+                    assert n == 0, f"view {i} steps={steps} render {k}: {n} records changed"
 
 
 # ---------------------------------------------------------------------------

@@ -27,39 +27,72 @@ def _frame_dict(f):
         "normal": f.normal.cpu().numpy(),
         "position": f.position.cpu().numpy(),
         "illumination": f.illumination.cpu().numpy(),
-        "steps": f.steps.cpu().numpy().view(np.uint32),
     }
+    if f.steps is not None:
+        out["steps"] = f.steps.cpu().numpy().view(np.uint32)
     if f.stats is not None:
         out["stats"] = f.stats.cpu().numpy().view(np.uint64)
     return out
 
 
+POISON = 0x7FA5A5A5  # a NaN as float32; no hit record, step count or colour has it
+
+
+def poison(f):
+    """Fills every image of a frame with POISON before a render, so that a
+    pixel the kernel leaves unwritten cannot pass on what the memory held
+    (a frame's memory is often the previous frame's, of the same view)."""
+    import torch
+    for name in ("shaded", "hits", "color", "normal", "position", "illumination", "steps",
+                 "display"):
+        t = getattr(f, name, None)
+        if t is not None:
+            t.view(-1).view(torch.int32).fill_(POISON)
+    return f
+
+
 def gpu_render(ctx, world, cam, W, H, max_p, max_s, flags=0, traversal=0, ambient=0.0,
                albedo=None, sun=S.SUN):
-    """Renders twice: with the statistics counters (the kernel variant that
-    counts every lookup) and without (the variant the benchmark times, with
-    its own traversal loops); compare() checks both against the oracle."""
+    """Renders three times: with the statistics counters (the kernel variant
+    that counts every lookup), without them (the variant the benchmark times,
+    with its own traversal loops; storing step counts keeps the free-region
+    test off), and without step counts with the free-region test forced on
+    (rays end early where they can hit nothing more: every output but the
+    step counts is unchanged).  compare() checks all three against the
+    oracle."""
     import torch
     prm = vt.params(W, H, max_p, max_s, sun, flags, traversal, ambient, albedo)
-    f = vt.Frame.alloc(W, H, gbuffer=True, steps=True, stats=True)
+    f = poison(vt.Frame.alloc(W, H, gbuffer=True, steps=True, stats=True))
     vt.render(ctx, world, cam, f, prm)
-    ff = vt.Frame.alloc(W, H, gbuffer=True, steps=True)
+    ff = poison(vt.Frame.alloc(W, H, gbuffer=True, steps=True))
     vt.render(ctx, world, cam, ff, prm)
+    prm_free = vt.params(W, H, max_p, max_s, sun, flags | vt.VT_FLAG_FREE_MAP_ON, traversal,
+                         ambient, albedo)
+    fr = poison(vt.Frame.alloc(W, H, gbuffer=True))
+    vt.render(ctx, world, cam, fr, prm_free)
     torch.cuda.synchronize()
     out = _frame_dict(f)
     out["fast"] = _frame_dict(ff)
+    out["free"] = _frame_dict(fr)
     return out
 
 
 def compare(g, o, rows=None, stats=True, label=""):
     if "fast" in g:
         compare(g["fast"], o, rows=rows, stats=False, label=label + " [no-stats kernel]")
+    if "free" in g:
+        compare(g["free"], o, rows=rows, stats=False, label=label + " [free-region test]")
     sl = slice(None) if rows is None else slice(*rows)
     print(f"[parity] {label}", flush=True)
     gh, oh = g["hits"][sl], o["hits"][sl]
     bad = np.any(gh != oh, axis=-1)
+    if bad.any():
+        first = [(tuple(int(i) for i in ix), [hex(int(v)) for v in gh[tuple(ix)]],
+                  [hex(int(v)) for v in oh[tuple(ix)]]) for ix in np.argwhere(bad)[:6]]
+        print(f"[parity] {label}: (pixel, gpu record, oracle record) {first}", flush=True)
     assert not bad.any(), f"{label}: {bad.sum()} hit records differ, first at {np.argwhere(bad)[:5]}"
-    assert np.array_equal(g["steps"][sl], o["steps"][sl]), f"{label}: step counts differ"
+    if "steps" in g:
+        assert np.array_equal(g["steps"][sl], o["steps"][sl]), f"{label}: step counts differ"
     for k in ("position", "normal", "illumination"):
         assert np.array_equal(g[k][sl].view(np.uint32), o[k][sl].view(np.uint32)), f"{label}: {k}"
     assert np.max(np.abs(g["color"][sl] - o["color"][sl])) <= RGB_TOL, f"{label}: color"

@@ -55,6 +55,9 @@ void free_world(vt_world* w) {
   dev_free(w->pool);
   dev_free(w->atlas);
   dev_free(w->occ);
+  dev_free(w->brick_bits);
+  dev_free(w->freemap);
+  dev_free(w->free_tmp);
   delete[] w->chunk_mirror;
   delete w;
 }
@@ -81,6 +84,16 @@ int dev_alloc(vt_world* w, void** p, size_t bytes, const char* what) {
   return VT_OK;
 }
 
+// The free-region map and its build scratch (built by vt::rebuild_freemap).
+int alloc_freemap(vt_world* w) {
+  const uint32_t R = vt::free_regions(w->dim);
+  int st;
+  if ((st = dev_alloc(w, (void**)&w->freemap, 8u * vt::free_words_per_octant(w->dim) * 4u,
+                      "free-region map")) != VT_OK)
+    return st;
+  return dev_alloc(w, (void**)&w->free_tmp, (size_t)8u * R * R * R, "free-region scratch");
+}
+
 }  // namespace
 
 namespace vt {
@@ -102,6 +115,9 @@ DenseWorld dense_view(const vt_world* w) {
   d.mz = (0x09249249u << 2) & low;
   d.nvox = w->dim * w->dim * w->dim;
   d.prefix = w->prefix;
+  d.freemap = w->freemap;
+  d.free_oct_bytes = free_words_per_octant(w->dim) * 4u;
+  d.free_bytes = w->freemap && w->free_valid ? 8u * d.free_oct_bytes : 0u;
   return d;
 }
 
@@ -119,6 +135,9 @@ BrickmapWorld brickmap_view(const vt_world* w) {
   b.mx = 0x09249249u & low;
   b.my = (0x09249249u << 1) & low;
   b.mz = (0x09249249u << 2) & low;
+  b.freemap = w->freemap;
+  b.free_oct_bytes = free_words_per_octant(w->dim) * 4u;
+  b.free_bytes = w->freemap && w->free_valid ? 8u * b.free_oct_bytes : 0u;
   return b;
 }
 
@@ -136,7 +155,10 @@ int check_params(const vt_render_params& p, const vt_world* world, const char* w
   if (p.max_steps_primary < 0 || p.max_steps_primary > 65535 || p.max_steps_shadow < 0 ||
       p.max_steps_shadow > 65535)
     return fail(VT_ERR_INVALID_ARGUMENT, "%s: step caps must lie in [0, 65535]", who);
-  if (p.flags & ~(VT_FLAG_GBUF_UNORM8 | VT_FLAG_LAMBERT | VT_FLAG_ALBEDO_COMPACT |
+  if ((p.flags & VT_FLAG_FREE_MAP_ON) && (p.flags & VT_FLAG_FREE_MAP_OFF))
+    return fail(VT_ERR_INVALID_ARGUMENT, "%s: FREE_MAP_ON and FREE_MAP_OFF together", who);
+  if (p.flags & ~(VT_FLAG_FREE_MAP_ON | VT_FLAG_FREE_MAP_OFF | VT_FLAG_GBUF_UNORM8 |
+                  VT_FLAG_LAMBERT | VT_FLAG_ALBEDO_COMPACT |
                   VT_FLAG_GRAD_L2 | VT_FLAG_PRESENT))
     return fail(VT_ERR_INVALID_ARGUMENT, "%s: unknown flags 0x%x", who, p.flags);
   if ((p.flags & VT_FLAG_PRESENT) && (p.flags & VT_FLAG_LAMBERT))
@@ -218,6 +240,19 @@ ShadeConsts shade_consts(const vt_render_params& p) {
   sc.grad_sun = p.grad_sun;
   sc.loss = p.loss;
   return sc;
+}
+
+bool render_free_map(const vt_world* w, const vt_render_params& p, const vt_frame& f) {
+  if (f.steps) return false;  // step counts are stored
+  if (w->kind == WorldKind::Dense) {
+    if (p.traversal != VT_TRAVERSAL_FLAT) return false;
+    // statistics count traceMap's steps unless the test is forced on (then
+    // the steps executed)
+    if (f.stats && !(p.flags & VT_FLAG_FREE_MAP_ON)) return false;
+  } else if (f.stats || !w->occ) {  // the reference layout's statistics kernel is the literal loop
+    return false;
+  }
+  return use_free_map(w, shade_consts(p));
 }
 
 FrameOut frame_out(const vt_frame* frame) {
@@ -339,7 +374,8 @@ int vt_world_create_dense_u8(vt_ctx* ctx, const uint8_t* grid, uint32_t dim,
       (st = dev_alloc(w, (void**)&w->l0, n / 8, "L0 masks")) != VT_OK ||
       (st = dev_alloc(w, (void**)&w->l1, n / 512, "L1 masks")) != VT_OK ||
       (st = dev_alloc(w, (void**)&w->l1b, vt::l1b_words(dim) * 8, "8^3-brick masks")) != VT_OK ||
-      (st = dev_alloc(w, (void**)&w->palette, 256 * 4, "palette")) != VT_OK) {
+      (st = dev_alloc(w, (void**)&w->palette, 256 * 4, "palette")) != VT_OK ||
+      (st = alloc_freemap(w)) != VT_OK) {
     free_world(w);
     return st;
   }
@@ -353,7 +389,9 @@ int vt_world_create_dense_u8(vt_ctx* ctx, const uint8_t* grid, uint32_t dim,
   e = hipMemcpy(lin, grid, n, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(w->palette, palette_rgba8, 256 * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = vt::launch_build_dense(lin, dim, w->vox, w->l0, w->l1, w->l1b, nullptr);
+  if (e == hipSuccess) e = vt::rebuild_freemap(w, nullptr);
   if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  if (e == hipSuccess) e = vt::finish_freemap(w);
   dev_free(lin);
   if (e != hipSuccess) {
     free_world(w);
@@ -415,12 +453,17 @@ int vt_world_create_brickmap(vt_ctx* ctx, uint32_t dim, const uint32_t* chunks,
   }
   if ((dim & (dim - 1)) == 0) {  // power-of-two map: occupancy bytes for the fast loop
     const size_t n = (size_t)dim * dim * dim;
-    if ((st = dev_alloc(w, (void**)&w->occ, n, "block occupancy")) != VT_OK) {
+    if ((st = dev_alloc(w, (void**)&w->occ, n, "block occupancy")) != VT_OK ||
+        (st = dev_alloc(w, (void**)&w->brick_bits, (size_t)vt::brick_bit_words(dim) * 8,
+                        "block brick bits")) != VT_OK ||
+        (st = alloc_freemap(w)) != VT_OK) {
       free_world(w);
       return st;
     }
     e = vt::launch_build_occupancy(vt::brickmap_view(w), w->occ, nullptr);
+    if (e == hipSuccess) e = vt::rebuild_freemap(w, nullptr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = vt::finish_freemap(w);
     if (e != hipSuccess) {
       free_world(w);
       return hip_fail(e, "vt_world_create_brickmap: occupancy");
@@ -441,6 +484,18 @@ int vt_world_info(const vt_world* world, uint32_t* dim, size_t* device_bytes) {
   if (!world) return fail(VT_ERR_NULL_POINTER, "vt_world_info: world is NULL");
   if (dim) *dim = world->dim;
   if (device_bytes) *device_bytes = world->bytes;
+  return VT_OK;
+}
+
+int vt_free_map_used(const vt_world* world, const vt_render_params* params,
+                     const vt_frame* frame, int* used) {
+  if (!world || !params || !frame || !used)
+    return fail(VT_ERR_NULL_POINTER, "vt_free_map_used: world, params, frame, used required");
+  *used = 0;
+  int st = vt::check_params(*params, world, "vt_free_map_used");
+  if (st == VT_OK) st = vt::check_frame(frame, params->flags, "vt_free_map_used");
+  if (st != VT_OK) return st;
+  *used = vt::render_free_map(world, *params, *frame) ? 1 : 0;
   return VT_OK;
 }
 
@@ -465,12 +520,18 @@ int vt_render(vt_ctx* ctx, const vt_world* world, const vt_camera* camera,
   const uint32_t blocks_8x8 = ((p.width + 15u) / 16u) * ((p.height + 15u) / 16u);
   vt::TileShape tile{blocks_8x8 >= 8u * (uint32_t)ctx->compute_units ? 4u : 3u};
   hipError_t e;
-  if (world->kind == WorldKind::Dense)
-    e = vt::launch_render_dense(vt::dense_view(world), p.traversal, v, sc, o, tile,
-                                (uint32_t)ctx->compute_units, (hipStream_t)stream);
-  else
-    e = vt::launch_render_brickmap(vt::brickmap_view(world), v, sc, o, tile,
-                                   (uint32_t)ctx->compute_units, (hipStream_t)stream);
+  const bool free_map = vt::render_free_map(world, p, *frame);
+  if (world->kind == WorldKind::Dense) {
+    vt::DenseWorld dw = vt::dense_view(world);
+    if (!free_map) dw.free_bytes = 0;
+    e = vt::launch_render_dense(dw, p.traversal, v, sc, o, tile, (uint32_t)ctx->compute_units,
+                                (hipStream_t)stream);
+  } else {
+    vt::BrickmapWorld bw = vt::brickmap_view(world);
+    if (!free_map) bw.free_bytes = 0;
+    e = vt::launch_render_brickmap(bw, v, sc, o, tile, (uint32_t)ctx->compute_units,
+                                   (hipStream_t)stream);
+  }
   if (e != hipSuccess) return hip_fail(e, "vt_render: launch");
   return VT_OK;
 }

@@ -153,6 +153,7 @@ struct vt_batch {
   vt::BatchView* d_views;
   uint32_t* d_queues;
   unsigned long long* stats;
+  bool steps_out;  // a view stores step counts: rays are not ended early
   hipStream_t cap_stream;
   hipGraph_t graph;
   hipGraphExec_t exec;
@@ -164,11 +165,14 @@ using vt::fail;
 using vt::hip_fail;
 
 // Queue reset + persistent launch over a device view table.
+// steps_out: some view stores step counts, which the free-region map's early
+// end of a ray would change, so the map is not read.
 hipError_t launch_batch(const vt_world* world, const vt::ShadeConsts& sc_in,
                         const vt::BatchView* d_views, uint32_t n_views, uint32_t total_tiles,
                         uint32_t* d_queues, const vt_tuning& tuning, uint32_t nblocks,
-                        unsigned long long* stats, hipStream_t s) {
-  const vt::DenseWorld w = vt::dense_view(world);
+                        unsigned long long* stats, bool steps_out, hipStream_t s) {
+  vt::DenseWorld w = vt::dense_view(world);
+  if (steps_out || stats || !vt::use_free_map(world, sc_in)) w.free_bytes = 0;
   vt::ShadeConsts sc = sc_in;
   sc.shadow = vt::make_ray_consts(sc.sun[0], sc.sun[1], sc.sun[2], w.mx, w.my, w.mz);
   hipLaunchKernelGGL(vt::k_batch_reset, dim3(1), dim3(64), 0, s, d_queues);
@@ -193,6 +197,7 @@ hipError_t launch_batch(const vt_world* world, const vt::ShadeConsts& sc_in,
     else VT_BATCH_SHADE(BitmaskLookup8, false)
   } else {
     if (stats) VT_BATCH_SHADE(FlatLookup, true)
+    else if (w.free_bytes) VT_BATCH_SHADE(FlatLookupFree, false)
     else VT_BATCH_SHADE(FlatLookup, false)
   }
 #undef VT_BATCH_SHADE
@@ -202,7 +207,7 @@ hipError_t launch_batch(const vt_world* world, const vt::ShadeConsts& sc_in,
 
 hipError_t enqueue(const vt_batch* b, hipStream_t s) {
   return launch_batch(b->world, b->sc, b->d_views, b->n_views, b->total_tiles, b->d_queues,
-                      b->tuning, b->nblocks, b->stats, s);
+                      b->tuning, b->nblocks, b->stats, b->steps_out, s);
 }
 
 // Validation shared by vt_batch_create and vt_batch_render_noalloc, and the
@@ -292,6 +297,7 @@ int vt_batch_create(vt_ctx* ctx, const vt_world* world, const vt_view* views, ui
   b->tuning = t;
   b->sc = vt::shade_consts(p);
   b->stats = stats;
+  for (uint32_t i = 0; i < n_views; i++) b->steps_out |= views[i].frame.steps != nullptr;
   b->nblocks = (uint32_t)ctx->compute_units * t.blocks_per_cu;
   hipError_t e = hipMalloc((void**)&b->d_views, sizeof(vt::BatchView) * n_views);
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_queues, sizeof(uint32_t) * vt::kQueues);
@@ -436,9 +442,12 @@ int vt_batch_render_noalloc(vt_ctx* ctx, const vt_world* world, const vt_view* v
     hipLaunchKernelGGL(vt::k_stage_views, dim3(1), dim3(64), 0, s, sp->views, pack);
     VT_HIP(hipGetLastError());
   }
+  bool steps_out = false;
+  for (uint32_t i = 0; i < n_views; i++) steps_out |= views[i].frame.steps != nullptr;
   const hipError_t e = launch_batch(world, vt::shade_consts(p), sp->views, n_views,
                                     (uint32_t)tiles, sp->queues, t,
-                                    (uint32_t)ctx->compute_units * t.blocks_per_cu, stats, s);
+                                    (uint32_t)ctx->compute_units * t.blocks_per_cu, stats,
+                                    steps_out, s);
   if (e != hipSuccess) return hip_fail(e, "vt_batch_render_noalloc: launch");
   return VT_OK;
 }

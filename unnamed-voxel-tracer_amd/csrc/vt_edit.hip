@@ -316,6 +316,7 @@ int set_brickmap(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint3
     hipLaunchKernelGGL(vt::k_scatter_u32, dim3(blocks_for(caddr.size(), 256)), dim3(256), 0, s,
                        w->chunks, d_caddr, d_cval, (uint32_t)caddr.size());
   hipError_t e = hipGetLastError();
+  if (e == hipSuccess && !oaddr.empty()) e = vt::rebuild_freemap(w, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     // Undo what may have reached the device: new chunk entries back to 0
@@ -333,6 +334,7 @@ int set_brickmap(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint3
   }
   for (auto& c : new_chunks) w->chunk_mirror[c.first] = c.second;
   w->pool_words = used * 512u;
+  if (!oaddr.empty()) VT_HIP(vt::finish_freemap(w));
   return VT_OK;
 }
 
@@ -384,6 +386,7 @@ int set_dense(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint32_t
                        dim3(256), 0, s, w->l0, w->l1, d_regions, (uint32_t)regions.size());
     VT_HIP(vt::launch_build_l1b(w->l0, (uint64_t)w->dim * w->dim * w->dim / 64u, d_regions8,
                                 (uint32_t)regions8.size(), w->l1b, s));
+    VT_HIP(vt::rebuild_freemap(w, s));
   }
   VT_HIP(hipGetLastError());
   if (w->prefix && !addr.empty()) {  // keep the compact index current
@@ -399,6 +402,7 @@ int set_dense(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint32_t
     w->n_occupied = total;
   }
   VT_HIP(hipStreamSynchronize(s));
+  if (!addr.empty()) VT_HIP(vt::finish_freemap(w));
   return VT_OK;
 }
 
@@ -482,10 +486,11 @@ int vt_pick(vt_ctx* ctx, const vt_world* world, const vt_camera* camera, int32_t
   const hipStream_t s = (hipStream_t)stream;
   const vt::ViewConsts v = vt::view_consts(camera, 1, 1);
   vt_pick_result* d = (vt_pick_result*)ctx->stage;  // the context's staging
-  if (world->kind == WorldKind::Dense)
-    hipLaunchKernelGGL(vt::k_pick_dense, dim3(1), dim3(64), 0, s, vt::dense_view(world), v,
-                       (int)max_steps, d);
-  else
+  if (world->kind == WorldKind::Dense) {
+    vt::DenseWorld dw = vt::dense_view(world);
+    dw.free_bytes = 0;  // the result reports traceMap's step count
+    hipLaunchKernelGGL(vt::k_pick_dense, dim3(1), dim3(64), 0, s, dw, v, (int)max_steps, d);
+  } else
     hipLaunchKernelGGL(vt::k_pick_brickmap, dim3(1), dim3(64), 0, s, vt::brickmap_view(world), v,
                        (int)max_steps, d);
   hipError_t e = hipGetLastError();

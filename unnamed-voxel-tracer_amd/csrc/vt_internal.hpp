@@ -45,6 +45,12 @@ struct vt_world {
   size_t pool_capacity;  // words the device pool holds
   uint32_t* chunk_mirror;  // host copy of the chunk table (edit allocation)
   uint8_t* occ;            // block occupancy in Morton order (power-of-two dim)
+  uint64_t* brick_bits;    // brickmap: bit per 4^3 brick of blocks, from occ
+  // free-region map (dense worlds and power-of-two brickmap worlds)
+  uint32_t* freemap;       // 8 octants x free_words_per_octant(dim) words
+  uint8_t* free_tmp;       // 8 x free_regions(dim)^3 bytes of build scratch
+  bool free_valid;         // the map matches the occupancy (finish_freemap)
+  float free_frac;         // largest share of free regions over the 8 octants
   // compact albedo index of a dense world (built on request)
   uint32_t* prefix;     // occupied voxels before each L0 word
   uint64_t n_occupied;
@@ -144,6 +150,40 @@ inline uint64_t l1b_words(uint32_t dim) {
 // from the L0 words, n0 of them.
 hipError_t launch_build_l1b(const uint64_t* l0, uint64_t n0, const uint32_t* words,
                             uint32_t nwords, uint64_t* l1b, hipStream_t stream);
+// Free-region map: regions of 16^3 blocks, R = free_regions(dim) per axis;
+// bit r of octant o's bitmap (free_words_per_octant(dim) words) is set when
+// no region at or ahead of r on every axis in o's travel direction holds a
+// nonempty block.  Built from the brick bits (a dense world's L1 words, a
+// brickmap world's brick_bits) with 8 R^3 bytes of scratch.
+uint32_t free_regions(uint32_t dim);
+uint32_t free_words_per_octant(uint32_t dim);
+hipError_t launch_build_freemap(const uint64_t* l1, uint32_t dim, uint8_t* tmp, uint32_t* fm,
+                                hipStream_t stream);
+// Rebuilds a world's free-region map from its current occupancy (a dense
+// world's L1 words; a brickmap world's occupancy bytes); no-op without one.
+// Clears free_valid: the caller calls finish_freemap once the stream has
+// synchronised without error, and until then the renderers do not read it.
+hipError_t rebuild_freemap(vt_world* w, hipStream_t stream);
+// After a rebuild completed: reads the map back to learn the largest share
+// of free regions over the octants (free_frac) and marks it valid.
+hipError_t finish_freemap(vt_world* w);
+// Whether a launch with these constants tests the free-region map: the map
+// is valid and has a free region, and VT_FLAG_FREE_MAP_ON, or neither
+// FREE_MAP flag, a step cap of at least VT_FREE_MIN_CAP and at least
+// 1/VT_FREE_MIN_SHARE of the regions free in some octant.  (Frames that
+// store step counts or statistics are excluded by the launchers.)
+bool use_free_map(const vt_world* w, const ShadeConsts& sc);
+// Whether vt_render makes the test for this world, parameters and frame:
+// use_free_map, and no stored step counts; on dense worlds the flat
+// traversal, and statistics only with VT_FLAG_FREE_MAP_ON (they then count
+// the steps executed); on the reference layout occupancy bytes and no
+// statistics.
+bool render_free_map(const vt_world* w, const vt_render_params& p, const vt_frame& f);
+// Brick bits (one uint64 per 4096 codes) of a Morton-ordered byte array of
+// dim^3 bytes; brick_bit_words(dim) words.
+uint32_t brick_bit_words(uint32_t dim);
+hipError_t launch_brick_bits(const uint8_t* bytes, uint32_t dim, uint64_t* l1,
+                             hipStream_t stream);
 // Exclusive prefix sum of popcount(l0[i]) into prefix[0..nwords); scratch
 // holds ceil(nwords/1024) + 1 words; *total receives the sum (device).
 hipError_t launch_compact_prefix(const uint64_t* l0, uint64_t nwords, uint32_t* prefix,

@@ -460,7 +460,7 @@ __device__ __forceinline__ void trace_pixel_dense(const DenseWorld& w, const Vie
     const int psteps = p.steps;
     if (!dense_begin_k<kStats>(r, lk, w, sox, soy, soz, sc.shadow, sc.max_shadow, h, cnt.l1f,
                                cnt.l0f)) {
-      if constexpr (L::kPrefetch && !kStats) {
+      if constexpr (L::kPrefetch && (!kStats || L::kFree)) {
         // the sun is the same for every lane: a uniform branch
         if (sc.sun[0] >= 0.0f && sc.sun[1] >= 0.0f && sc.sun[2] >= 0.0f)
           dense_run_prefetch<L, true>(r, lk, w, h);
@@ -487,126 +487,5 @@ __device__ __forceinline__ void trace_pixel_dense(const DenseWorld& w, const Vie
                      gacc);
 }
 
-
-// A shadow ray still running after its first segment, parked in LDS until
-// the block's long shadow rays are dealt out to its waves (k_render_dense).
-// Its DDA state at an iteration start (every lookup so far empty) plus what
-// the pixel's stores need.
-struct ShadowJob {
-  float wx, wy, wz;
-  uint32_t c;
-  int steps;
-  uint32_t pxy;  // px | py << 16
-  uint32_t data, info0;
-  int psteps;
-  float sox, soy, soz;
-};
-
-// The first segment: one iteration of the empty-space phase (the phase runs
-// an iteration while steps + VT_SKIP_STEPS < limit).  It separates the
-// shadow rays occluded within a few steps (most of the occluded ones) from
-// the ones that run on, usually to the grid's edge.
-constexpr int kShadowSegment = VT_SKIP_STEPS + 1;
-
-template <class L>
-__device__ __forceinline__ bool shadow_seg(DenseRay& r, L& lk, const DenseWorld& w, DenseHit& h,
-                                           const ShadeConsts& sc, int seg) {
-  if (sc.sun[0] >= 0.0f && sc.sun[1] >= 0.0f && sc.sun[2] >= 0.0f)
-    return dense_run_prefetch_seg<L, true>(r, lk, w, h, seg);
-  return dense_run_prefetch_seg<L, false>(r, lk, w, h, seg);
-}
-
-// trace_pixel_dense for the reference composite (flat lookup, no
-// statistics), the shadow ray cut after its first segment: returns true
-// (and fills job) when the shadow ray is still running; every other case is
-// finished and stored here, exactly as trace_pixel_dense does.
-template <class L>
-__device__ __forceinline__ bool trace_pixel_dense_split(const DenseWorld& w, const ViewConsts& v,
-                                                        const ShadeConsts& sc, const FrameOut& out,
-                                                        uint32_t px, uint32_t py, Counters& cnt,
-                                                        ShadowJob& job) {
-  DenseRay r;
-  L lk;
-  DenseHit h;
-  Primary p;
-  float dx, dy, dz;
-  raygen(v, px, py, dx, dy, dz);
-  float tn, tf;
-  const float D = (float)w.dim;
-  intersect_aabb(v.pos[0], v.pos[1], v.pos[2], dx, dy, dz, 0.0f, 0.0f, 0.0f, D, D, D, tn, tf);
-  float tc = fmaxf(tn, 0.0f);
-  float ex = (v.pos[0] + dx * tc) - 0.001f;
-  float ey = (v.pos[1] + dy * tc) - 0.001f;
-  float ez = (v.pos[2] + dz * tc) - 0.001f;
-  if (!dense_begin<false>(r, lk, w, ex, ey, ez, dx, dy, dz, sc.max_primary, h, cnt.l1f, cnt.l0f))
-    dense_run<false>(r, lk, w, h, cnt.l1f, cnt.l0f);
-  p.hit = h.hit;
-  p.steps = h.steps;
-  if (!p.hit) {
-    GradAcc gacc = grad_acc();
-    write_pixel<kShadeRef>(v, sc, out, px, py, p, false, -1.0f, -1.0f, -1.0f, 0.f, 0.f, 0.f, 0,
-                           cnt, &w, gacc);
-    return false;
-  }
-  p.face = h.face;
-  p.voxel = (uint32_t)h.bx + w.dim * ((uint32_t)h.by + w.dim * (uint32_t)h.bz);
-  p.material = lk.material(w, h.code, h.v, cnt.leaf);
-  p.data = w.palette[p.material];
-  p.sub = h.sub;
-  p.code = h.code;
-  p.hx = h.hx;
-  p.hy = h.hy;
-  p.hz = h.hz;
-  float sox, soy, soz;
-  shadow_origin(p, sc.flags, sox, soy, soz);
-  store_hit_primary(v, sc, out, (size_t)py * v.W + px, p);
-  const uint32_t data = p.data, info0 = hit_info(p.face, p.sub, p.material, false);
-  const int psteps = p.steps;
-  // (the packed kernel measured 0.7% faster with the sun's constants made
-  // per lane than read from the launch's arguments)
-  bool done = dense_begin<false>(r, lk, w, sox, soy, soz, sc.sun[0], sc.sun[1], sc.sun[2],
-                                 sc.max_shadow, h, cnt.l1f, cnt.l0f);
-  if (!done) done = shadow_seg<L>(r, lk, w, h, sc, kShadowSegment);
-  if (done) {
-    store_hit_shadow_ref(v, sc, out, px, py, data, info0, psteps, h.hit, h.hit ? h.hx : -1.0f,
-                         h.hit ? h.hy : -1.0f, h.hit ? h.hz : -1.0f, sox, soy, soz, h.steps, cnt);
-    return false;
-  }
-  job.wx = r.wx;
-  job.wy = r.wy;
-  job.wz = r.wz;
-  job.c = r.c;
-  job.steps = r.steps;
-  job.pxy = px | (py << 16);
-  job.data = data;
-  job.info0 = info0;
-  job.psteps = psteps;
-  job.sox = sox;
-  job.soy = soy;
-  job.soz = soz;
-  return true;
-}
-
-// The rest of a parked shadow ray from its state, then the pixel's shadow
-// stores: the same steps and results as the uncut ray.
-template <class L>
-__device__ __forceinline__ void finish_shadow_job(const DenseWorld& w, const ViewConsts& v,
-                                                  const ShadeConsts& sc, const FrameOut& out,
-                                                  const ShadowJob& job, Counters& cnt) {
-  DenseRay r;
-  L lk;
-  DenseHit h;
-  dense_ray_consts(r, w, sc.sun[0], sc.sun[1], sc.sun[2]);
-  r.max_steps = sc.max_shadow;
-  r.wx = job.wx;
-  r.wy = job.wy;
-  r.wz = job.wz;
-  r.c = job.c;
-  r.steps = job.steps;
-  (void)shadow_seg<L>(r, lk, w, h, sc, 0x7FFFFFFF);
-  store_hit_shadow_ref(v, sc, out, job.pxy & 0xFFFFu, job.pxy >> 16, job.data, job.info0,
-                       job.psteps, h.hit, h.hit ? h.hx : -1.0f, h.hit ? h.hy : -1.0f,
-                       h.hit ? h.hz : -1.0f, job.sox, job.soy, job.soz, h.steps, cnt);
-}
 
 }  // namespace vt

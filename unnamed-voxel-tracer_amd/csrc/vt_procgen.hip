@@ -398,6 +398,7 @@ int vt_world_procgen(vt_ctx* ctx, vt_world* world, float offset_x, float offset_
   if (e == hipSuccess && world->occ)
     e = vt::launch_build_occupancy(vt::brickmap_view(world), world->occ, s,
                                    std::min(dim, (walk.y_end + 7u) & ~7u));
+  if (e == hipSuccess) e = vt::rebuild_freemap(world, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     // back to the empty map the call was given: chunk table, written pool
@@ -411,6 +412,7 @@ int vt_world_procgen(vt_ctx* ctx, vt_world* world, float offset_x, float offset_
   }
   std::memcpy(world->chunk_mirror, walk.mirror.data(), nchunks * 4);
   world->pool_words = walk.used * 512u;
+  VT_HIP(vt::finish_freemap(world));
   return VT_OK;
 }
 

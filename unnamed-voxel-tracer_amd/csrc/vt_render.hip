@@ -14,32 +14,12 @@
 // no XCD is left with the costliest rows.  Each lane traces its primary ray,
 // then (on a hit) its shadow ray.
 //
-// Shadow rays come in two kinds: most of those not lit are occluded after a
-// step or two, most lit ones run to the edge of the grid (hundreds of steps).
-// In a wave that holds both, the occluded lanes idle while the lit ones run.
-// So (reference composite, flat lookup) each lane runs its shadow ray for a
-// first segment of one empty-space iteration; the rays still running are
-// then parked in LDS, packed, and dealt out to the block's waves from the
-// first on, 64 to a wave, which finish them -- the block's waves without a
-// parked ray end there.  Same steps and results per ray; one block-wide
-// barrier on each side of the packing.  (A second kernel over the long rays
-// of the whole frame, through a job list in memory, was built and measured
-// slower than the single kernel once the XCD strips were in.)
+// Rays end early where nothing is left to hit: with the free-region test
+// (a per-world bitmap of the 16^3 regions from which every region ahead in a
+// direction octant is empty) a ray standing in such a region is a miss.  Lit
+// shadow rays, which otherwise run to the edge of the grid, mostly end a few
+// regions after their start, so the lanes of a wave finish close together.
 #include "vt_pixel.hpp"
-
-// Frames with at least this many 256-thread blocks per CU, and a shadow cap
-// of at least VT_PACK_MIN_SHADOW_CAP steps, take the packed shadow rays
-// (kPack of k_render_dense and k_render_brickmap).  The packing costs two
-// block-wide barriers: it pays when the frame fills every CU's block slots
-// (1080p faster, 512x512 views slower) and when shadow rays can run long --
-// the imbalance it removes is bounded by the shadow cap (the engine's own
-// frame, cap 48, measured 16% slower with it in the brickmap kernel).
-#ifndef VT_PACK_BLOCKS_PER_CU
-#define VT_PACK_BLOCKS_PER_CU 8u
-#endif
-#ifndef VT_PACK_MIN_SHADOW_CAP
-#define VT_PACK_MIN_SHADOW_CAP 256
-#endif
 
 namespace vt {
 
@@ -47,9 +27,7 @@ namespace vt {
 // per SIMD hide the gathers of the traversal loop; the spills it costs lie
 // outside it; 1080p 0.785 vs 0.804 ms at seven), the Lambert modes at most
 // 72 (seven waves: C3's Lambert forward 0.117 vs 0.120 ms per view).
-// kPack: the block packs its long shadow rays (reference composite, flat
-// lookup, no statistics); the launcher picks it for frames that fill the GPU.
-template <class L, bool kStats, int kShade, bool kPack = false>
+template <class L, bool kStats, int kShade>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(kShade == kShadeRef ? 8 : 7))) void k_render_dense(
     DenseWorld w, ViewConsts v, ShadeConsts sc, FrameOut out, uint32_t log2_tw,
@@ -61,28 +39,6 @@ __attribute__((amdgpu_waves_per_eu(kShade == kShadeRef ? 8 : 7))) void k_render_
   const uint32_t px = bx * (2u * tw) + (wave & 1u) * tw + (lane & (tw - 1u));
   const uint32_t py = by * (2u * th) + (wave >> 1) * th + (lane >> log2_tw);
   Counters cnt = {0, 0, 0, 0, 0, 0, 0, 0};
-  if constexpr (kPack && kShade == kShadeRef && L::kPrefetch && !kStats) {
-    // every thread of the block reaches both barriers: no early exit above
-    __shared__ ShadowJob jobs[256];
-    __shared__ uint32_t wave_jobs[4];
-    ShadowJob job;
-    bool surv = false;
-    if (px < v.W && py < v.H) surv = trace_pixel_dense_split<L>(w, v, sc, out, px, py, cnt, job);
-    const unsigned long long bal = __ballot(surv);
-    if (lane == 0) wave_jobs[wave] = (uint32_t)__popcll(bal);
-    __syncthreads();
-    uint32_t base = 0, total = 0;
-#pragma unroll
-    for (uint32_t i = 0; i < 4; i++) {
-      const uint32_t n = wave_jobs[i];
-      base += i < wave ? n : 0u;
-      total += n;
-    }
-    if (surv) jobs[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = job;
-    __syncthreads();
-    if (threadIdx.x < total) finish_shadow_job<L>(w, v, sc, out, jobs[threadIdx.x], cnt);
-    return;
-  }
   GradAcc gacc = grad_acc();
   if (px < v.W && py < v.H)
     trace_pixel_dense<L, kStats, kShade>(w, v, sc, out, px, py, cnt, gacc);
@@ -97,10 +53,8 @@ __attribute__((amdgpu_waves_per_eu(kShade == kShadeRef ? 8 : 7))) void k_render_
 // 0.460 ms at eight (fewer spills around its many literal-loop iterations),
 // the C1 world in this layout 0.823 vs 0.820 ms; five waves (90 VGPRs) were
 // slower on both.
-// kPack: the block packs its long shadow rays after a first segment and
-// finishes them 64 to a wave, as k_render_dense does (worlds with occupancy
-// bytes, no statistics).
-template <bool kStats, bool kPack = false>
+// kFree: the empty-space loops test the free-region map.
+template <bool kStats, bool kFree = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_render_brickmap(
     BrickmapWorld w, ViewConsts v, ShadeConsts sc, FrameOut out, uint32_t log2_tw,
     uint32_t blocks_x) {
@@ -112,110 +66,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
   const uint32_t py = by * (2u * th) + (wave >> 1) * th + (lane >> log2_tw);
   const bool active = px < v.W && py < v.H;
   Counters cnt = {0, 0, 0, 0, 0, 0, 0, 0};
-  if constexpr (kPack && !kStats) {
-    // (launched only for worlds with occupancy bytes) every thread of the
-    // block reaches both barriers
-    __shared__ ShadowJob jobs[256];
-    __shared__ uint32_t wave_jobs[4];
-    ShadowJob job;
-    bool surv = false;
-    const bool pos = sc.sun[0] >= 0.0f && sc.sun[1] >= 0.0f && sc.sun[2] >= 0.0f;
-    if (active) {
-      float dx, dy, dz;
-      raygen(v, px, py, dx, dy, dz);
-      float tn, tf;
-      const float D = (float)w.dim;
-      intersect_aabb(v.pos[0], v.pos[1], v.pos[2], dx, dy, dz, 0.0f, 0.0f, 0.0f, D, D, D, tn, tf);
-      float tc = fmaxf(tn, 0.0f);
-      float ex = (v.pos[0] + dx * tc) - 0.001f;
-      float ey = (v.pos[1] + dy * tc) - 0.001f;
-      float ez = (v.pos[2] + dz * tc) - 0.001f;
-      GeneralHit g;
-      trace_brickmap_fast(w, ex, ey, ez, dx, dy, dz, sc.max_primary, g);
-      Primary p;
-      p.hit = g.data != 0u;
-      p.steps = g.steps;
-      if (p.hit) {
-        p.face = g.face;
-        p.voxel = (uint32_t)g.bx + w.dim * ((uint32_t)g.by + w.dim * (uint32_t)g.bz);
-        p.data = g.data;
-        p.material = bm_block(w, g.bx, g.by, g.bz) & 255u;
-        p.sub = g.sub;
-        p.hx = g.hx;
-        p.hy = g.hy;
-        p.hz = g.hz;
-        float sox, soy, soz;
-        shadow_origin(p, sc.flags, sox, soy, soz);
-        store_hit_primary(v, sc, out, (size_t)py * v.W + px, p);
-        const uint32_t data = p.data, info0 = hit_info(p.face, p.sub, p.material, false);
-        GeneralHit sh;
-        BrickPark park;
-        if (pos)
-          trace_brickmap_fast<true, true>(w, sox, soy, soz, sc.sun[0], sc.sun[1], sc.sun[2],
-                                          sc.max_shadow, sh, &park);
-        else
-          trace_brickmap_fast<false, true>(w, sox, soy, soz, sc.sun[0], sc.sun[1], sc.sun[2],
-                                           sc.max_shadow, sh, &park);
-        if (sh.steps < 0) {
-          surv = true;
-          job.wx = park.wx;
-          job.wy = park.wy;
-          job.wz = park.wz;
-          job.c = park.c;
-          job.steps = park.steps;
-          job.pxy = px | (py << 16);
-          job.data = data;
-          job.info0 = info0;
-          job.psteps = p.steps;
-          job.sox = sox;
-          job.soy = soy;
-          job.soz = soz;
-        } else {
-          const bool shit = sh.data != 0u;
-          store_hit_shadow_ref(v, sc, out, px, py, data, info0, p.steps, shit,
-                               shit ? sh.hx : -1.0f, shit ? sh.hy : -1.0f, shit ? sh.hz : -1.0f,
-                               sox, soy, soz, sh.steps, cnt);
-        }
-      } else {
-        GradAcc gacc = grad_acc();
-        write_pixel<kShadeRef>(v, sc, out, px, py, p, false, -1.0f, -1.0f, -1.0f, 0.f, 0.f, 0.f, 0,
-                               cnt, nullptr, gacc);
-      }
-    }
-    const unsigned long long bal = __ballot(surv);
-    if (lane == 0) wave_jobs[wave] = (uint32_t)__popcll(bal);
-    __syncthreads();
-    uint32_t base = 0, total = 0;
-#pragma unroll
-    for (uint32_t i = 0; i < 4; i++) {
-      const uint32_t n = wave_jobs[i];
-      base += i < wave ? n : 0u;
-      total += n;
-    }
-    if (surv) jobs[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = job;
-    __syncthreads();
-    if (threadIdx.x < total) {
-      const ShadowJob& jb = jobs[threadIdx.x];
-      BrickPark from;
-      from.wx = jb.wx;
-      from.wy = jb.wy;
-      from.wz = jb.wz;
-      from.c = jb.c;
-      from.steps = jb.steps;
-      GeneralHit sh;
-      if (pos)
-        trace_brickmap_fast<true, false, true>(w, 0.f, 0.f, 0.f, sc.sun[0], sc.sun[1], sc.sun[2],
-                                               sc.max_shadow, sh, nullptr, &from);
-      else
-        trace_brickmap_fast<false, false, true>(w, 0.f, 0.f, 0.f, sc.sun[0], sc.sun[1],
-                                                sc.sun[2], sc.max_shadow, sh, nullptr, &from);
-      const bool shit = sh.data != 0u;
-      store_hit_shadow_ref(v, sc, out, jb.pxy & 0xFFFFu, jb.pxy >> 16, jb.data, jb.info0,
-                           jb.psteps, shit, shit ? sh.hx : -1.0f, shit ? sh.hy : -1.0f,
-                           shit ? sh.hz : -1.0f, jb.sox, jb.soy, jb.soz, sh.steps, cnt);
-    }
-    return;
-  }
   if (active) {
     float dx, dy, dz;
     raygen(v, px, py, dx, dy, dz);
@@ -228,7 +78,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     float ez = (v.pos[2] + dz * tc) - 0.001f;
     GeneralHit g;
     if (!kStats && w.occ)
-      trace_brickmap_fast(w, ex, ey, ez, dx, dy, dz, sc.max_primary, g);
+      trace_brickmap_fast<false, kFree>(w, ex, ey, ez, dx, dy, dz, sc.max_primary, g);
     else
       trace_general<kStats>(w, ex, ey, ez, dx, dy, dz, sc.max_primary, g, cnt.l1f, cnt.l0f);
     Primary p;
@@ -255,11 +105,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
       if (!kStats && w.occ) {
         // the sun is the same for every lane: a uniform branch
         if (sc.sun[0] >= 0.0f && sc.sun[1] >= 0.0f && sc.sun[2] >= 0.0f)
-          trace_brickmap_fast<true>(w, sox, soy, soz, sc.sun[0], sc.sun[1], sc.sun[2],
-                                    sc.max_shadow, s);
+          trace_brickmap_fast<true, kFree>(w, sox, soy, soz, sc.sun[0], sc.sun[1], sc.sun[2],
+                                           sc.max_shadow, s);
         else
-          trace_brickmap_fast<false>(w, sox, soy, soz, sc.sun[0], sc.sun[1], sc.sun[2],
-                                     sc.max_shadow, s);
+          trace_brickmap_fast<false, kFree>(w, sox, soy, soz, sc.sun[0], sc.sun[1], sc.sun[2],
+                                            sc.max_shadow, s);
       } else {
         trace_general<kStats>(w, sox, soy, soz, sc.sun[0], sc.sun[1], sc.sun[2], sc.max_shadow,
                               s, cnt.l1f, cnt.l0f);
@@ -293,7 +143,7 @@ static void grid_shape(const ViewConsts& v, TileShape tile, uint32_t& blocks_x, 
 template <class L, bool kStats>
 static void launch_dense_shade(const DenseWorld& w, const ViewConsts& v, const ShadeConsts& sc,
                                const FrameOut& out, uint32_t log2_w, uint32_t blocks_x, dim3 grid,
-                               bool pack, hipStream_t stream) {
+                               hipStream_t stream) {
   switch (shade_mode(sc.flags)) {
     case kShadeLambertGrad:
       hipLaunchKernelGGL((k_render_dense<L, kStats, kShadeLambertGrad>), grid, dim3(256), 0,
@@ -304,59 +154,56 @@ static void launch_dense_shade(const DenseWorld& w, const ViewConsts& v, const S
                          v, sc, out, log2_w, blocks_x);
       break;
     default:
-      if constexpr (L::kPrefetch && !kStats) {
-        if (pack) {
-          hipLaunchKernelGGL((k_render_dense<L, kStats, kShadeRef, true>), grid, dim3(256), 0,
-                             stream, w, v, sc, out, log2_w, blocks_x);
-          break;
-        }
-      }
       hipLaunchKernelGGL((k_render_dense<L, kStats, kShadeRef>), grid, dim3(256), 0, stream, w, v,
                          sc, out, log2_w, blocks_x);
   }
 }
 
-hipError_t launch_render_dense(const DenseWorld& w, uint32_t traversal, const ViewConsts& v,
+hipError_t launch_render_dense(const DenseWorld& w_in, uint32_t traversal, const ViewConsts& v,
                                const ShadeConsts& sc_in, const FrameOut& out, TileShape tile,
                                uint32_t compute_units, hipStream_t stream) {
+  (void)compute_units;
+  DenseWorld w = w_in;
+  if (out.steps) w.free_bytes = 0;  // step counts are stored: no early end of rays
   ShadeConsts sc = sc_in;
   sc.shadow = make_ray_consts(sc.sun[0], sc.sun[1], sc.sun[2], w.mx, w.my, w.mz);
   uint32_t blocks_x, nblocks;
   grid_shape(v, tile, blocks_x, nblocks);
-  const bool pack = nblocks >= VT_PACK_BLOCKS_PER_CU * compute_units &&
-                    sc.max_shadow >= VT_PACK_MIN_SHADOW_CAP;
   const dim3 grid(nblocks);
   const bool stats = out.stats != nullptr;
+  const uint32_t lw = tile.log2_w;
   if (traversal == VT_TRAVERSAL_BITMASK) {
-    if (stats)
-      launch_dense_shade<BitmaskLookup, true>(w, v, sc, out, tile.log2_w, blocks_x, grid, pack, stream);
-    else
-      launch_dense_shade<BitmaskLookup, false>(w, v, sc, out, tile.log2_w, blocks_x, grid, pack, stream);
+    if (stats) launch_dense_shade<BitmaskLookup, true>(w, v, sc, out, lw, blocks_x, grid, stream);
+    else launch_dense_shade<BitmaskLookup, false>(w, v, sc, out, lw, blocks_x, grid, stream);
   } else if (traversal == VT_TRAVERSAL_BITMASK8) {
-    if (stats)
-      launch_dense_shade<BitmaskLookup8, true>(w, v, sc, out, tile.log2_w, blocks_x, grid, pack, stream);
-    else
-      launch_dense_shade<BitmaskLookup8, false>(w, v, sc, out, tile.log2_w, blocks_x, grid, pack, stream);
+    if (stats) launch_dense_shade<BitmaskLookup8, true>(w, v, sc, out, lw, blocks_x, grid, stream);
+    else launch_dense_shade<BitmaskLookup8, false>(w, v, sc, out, lw, blocks_x, grid, stream);
   } else {
-    if (stats)
-      launch_dense_shade<FlatLookup, true>(w, v, sc, out, tile.log2_w, blocks_x, grid, pack, stream);
-    else
-      launch_dense_shade<FlatLookup, false>(w, v, sc, out, tile.log2_w, blocks_x, grid, pack, stream);
+    if (stats && w.free_bytes)  // statistics of the steps executed
+      launch_dense_shade<FlatLookupFree, true>(w, v, sc, out, lw, blocks_x, grid, stream);
+    else if (stats) launch_dense_shade<FlatLookup, true>(w, v, sc, out, lw, blocks_x, grid, stream);
+    else if (w.free_bytes)
+      launch_dense_shade<FlatLookupFree, false>(w, v, sc, out, lw, blocks_x, grid, stream);
+    else launch_dense_shade<FlatLookup, false>(w, v, sc, out, lw, blocks_x, grid, stream);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_render_brickmap(const BrickmapWorld& w, const ViewConsts& v,
+hipError_t launch_render_brickmap(const BrickmapWorld& w_in, const ViewConsts& v,
                                   const ShadeConsts& sc, const FrameOut& out, TileShape tile,
                                   uint32_t compute_units, hipStream_t stream) {
+  (void)compute_units;
+  BrickmapWorld w = w_in;
+  // step counts stored, or statistics (of traceMap's steps on this layout):
+  // no early end of rays
+  if (out.steps || out.stats) w.free_bytes = 0;
   uint32_t blocks_x, nblocks;
   grid_shape(v, tile, blocks_x, nblocks);
   dim3 grid(nblocks), block(256);
   if (out.stats)
     hipLaunchKernelGGL((k_render_brickmap<true>), grid, block, 0, stream, w, v, sc, out,
                        tile.log2_w, blocks_x);
-  else if (w.occ && nblocks >= VT_PACK_BLOCKS_PER_CU * compute_units &&
-           sc.max_shadow >= VT_PACK_MIN_SHADOW_CAP)
+  else if (w.occ && w.free_bytes)
     hipLaunchKernelGGL((k_render_brickmap<false, true>), grid, block, 0, stream, w, v, sc, out,
                        tile.log2_w, blocks_x);
   else

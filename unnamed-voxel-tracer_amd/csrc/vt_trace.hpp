@@ -207,6 +207,9 @@ struct DenseWorld {
   uint32_t nvox;            // dim^3 (bytes of vox)
   const uint32_t* prefix;   // compact albedo index: occupied voxels before each
                             // L0 word (null until built)
+  // free-region map (see FreeMap below); free_bytes 0 = rays are not ended early
+  const uint32_t* freemap;
+  uint32_t free_bytes, free_oct_bytes;
 };
 
 __device__ __forceinline__ uint32_t part1by2(uint32_t x) {
@@ -237,6 +240,7 @@ __device__ __forceinline__ bool in_dim(const DenseWorld& w, int bx, int by, int 
 struct FlatLookup {
   // one byte gather per probe; returns the material (0 = empty)
   static constexpr bool kPrefetch = true;  // probes have no state: run ahead
+  static constexpr bool kFree = false;     // no free-region test
   __device__ __forceinline__ void reset() {}
   template <bool kStats>
   __device__ __forceinline__ uint32_t probe(const DenseWorld& w, uint32_t code, uint32_t&,
@@ -263,6 +267,7 @@ template <uint32_t kBrick>
 struct BitmaskLookupT {
   static_assert(kBrick == 4 || kBrick == 8, "4^3 or 8^3 bricks");
   static constexpr bool kPrefetch = false;
+  static constexpr bool kFree = false;
   static constexpr uint32_t kBrickShift = kBrick == 8 ? 9u : 6u;  // log2 codes per brick
   uint32_t brick, region, leaf;
   uint64_t bword, rword;
@@ -308,6 +313,12 @@ struct BitmaskLookupT {
 };
 using BitmaskLookup = BitmaskLookupT<4>;
 using BitmaskLookup8 = BitmaskLookupT<8>;
+
+// The flat lookup whose empty-space loops also test the free-region map (see
+// free_rsrc below) and end a ray that can hit nothing more.
+struct FlatLookupFree : FlatLookup {
+  static constexpr bool kFree = true;
+};
 
 // Result of one traceMap call in a dense world.
 struct DenseHit {
@@ -611,6 +622,51 @@ __device__ __forceinline__ void dense_loop_prefetch(
 #define VT_SKIP_STEPS 5
 #endif
 
+// How an empty-space run ended.
+enum SkipStatus : int {
+  kSkipLimit = 0,    // reached its step limit; every lookup so far empty
+  kSkipBlocked = 1,  // an iteration met a nonzero byte or left the grid
+  kSkipFree = 2,     // the ray is in a free region: it can hit nothing more
+};
+
+// The free-region map (built by launch_build_freemap): regions of 16^3
+// blocks, region code = block Morton code >> 12; bit r of the bitmap of
+// direction octant o is set when no region at or ahead of r on every axis in
+// o's travel direction (r included) holds a nonempty block.
+//
+// Why a ray standing in such a region can hit nothing more, so that ending
+// it as a miss changes no output but its step count: a step of traceMap's
+// block-level recurrence moves the crossed axis one block in the travel
+// direction and no axis the other way, and a lookup probes the state's block
+// G or, on a positive axis whose within reached 8, the block after it.  So
+// every later lookup is a block at or ahead of G on every axis -- inside the
+// forward orthant of G's region, or outside the grid, which reads as empty --
+// whatever the rounding and the 0.999 re-anchoring do to within.  With that
+// orthant empty every later lookup reads 0, and traceMap ends by leaving the
+// grid or at its step cap: a miss either way.  Octant bit k = the ray
+// travels negatively on axis k; free_bytes 0 reads as "never free" through
+// the bounds-checked load.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t free_rsrc(const uint32_t* fm, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)fm, (short)0, (int)bytes, 0x00020000);
+}
+
+// Byte offset of the octant bitmap of a ray whose complemented (negative)
+// axes are `flip`.
+__device__ __forceinline__ uint32_t free_octant_offset(uint32_t flip, uint32_t mx, uint32_t my,
+                                                       uint32_t mz, uint32_t oct_bytes) {
+  const uint32_t o = ((flip & mx) ? 1u : 0u) | ((flip & my) ? 2u : 0u) | ((flip & mz) ? 4u : 0u);
+  return o * oct_bytes;
+}
+
+// The free-map word holding region (block code >> 12) of real block code g.
+__device__ __forceinline__ uint32_t free_word(__amdgpu_buffer_rsrc_t fm, uint32_t fofs,
+                                             uint32_t g) {
+  return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(fm, (int)(fofs + ((g >> 17) << 2)), 0, 0);
+}
+__device__ __forceinline__ bool free_bit(uint32_t word, uint32_t g) {
+  return ((word >> ((g >> 12) & 31u)) & 1u) != 0u;
+}
+
 // Empty-space phase of the flat loop.  An iteration takes K block-level steps
 // from `s` (whose own lookup was empty, `steps` steps done) and requests all K
 // bytes without testing any of them in between; only when all K are zero and
@@ -622,21 +678,27 @@ __device__ __forceinline__ void dense_loop_prefetch(
 // K steps instead of one per step, and with it none of the per-step copies of
 // the exit state that a divergent exit costs.  An iteration only starts when
 // steps + K < limit (limit <= max_steps), so the step cap falls inside the
-// replay.  Returns true when it stopped at an iteration that met a nonzero
-// byte or left the grid, false when it stopped at the limit; either way `s`
-// is a state the exact loop (or another call) continues from.
-template <int K>
-__device__ __forceinline__ bool dense_skip_empty(float Px, float Py, float Pz, float Ax, float Ay,
-                                                 float Az, float dx, float dy, float dz,
-                                                 float ix, float iy, float iz, uint32_t mx,
-                                                 uint32_t my, uint32_t mz, uint32_t flip,
-                                                 DenseCell& s, int& steps, int limit,
-                                                 __amdgpu_buffer_rsrc_t bytes) {
+// replay.  Each iteration also requests the free-map word of its start
+// block and, when that region is free for the ray's octant (bitmap at byte
+// `fofs` of `fm`), ends the run with kSkipFree: the ray can hit nothing
+// more.  Returns kSkipBlocked when it stopped at an iteration that met a
+// nonzero byte or left the grid, kSkipLimit at the limit; either way `s` is
+// a state the exact loop (or another call) continues from.
+template <int K, bool kFree>
+__device__ __forceinline__ int dense_skip_empty(float Px, float Py, float Pz, float Ax, float Ay,
+                                                float Az, float dx, float dy, float dz, float ix,
+                                                float iy, float iz, uint32_t mx, uint32_t my,
+                                                uint32_t mz, uint32_t flip, DenseCell& s,
+                                                int& steps, int limit,
+                                                __amdgpu_buffer_rsrc_t bytes,
+                                                __amdgpu_buffer_rsrc_t fm, uint32_t fofs) {
   float bx = s.wx, by = s.wy, bz = s.wz;
   uint32_t bc = s.c;
   int sb = steps;
-  bool blocked = false;
+  int status = kSkipLimit;
   while (sb + K < limit) {
+    const uint32_t g = bc ^ flip;
+    const uint32_t fw = kFree ? free_word(fm, fofs, g) : 0u;
     DenseCell n;
     n.wx = bx;
     n.wy = by;
@@ -650,8 +712,12 @@ __device__ __forceinline__ bool dense_skip_empty(float Px, float Py, float Pz, f
       any |= (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(bytes, (int)n.a, 0, 0);
       wrapped |= n.wrapped;
     }
+    if (kFree && free_bit(fw, g)) {
+      status = kSkipFree;
+      break;
+    }
     if (wrapped | (any != 0u)) {
-      blocked = true;
+      status = kSkipBlocked;
       break;
     }
     bx = n.wx;
@@ -665,7 +731,7 @@ __device__ __forceinline__ bool dense_skip_empty(float Px, float Py, float Pz, f
   s.wz = bz;
   s.c = bc;
   steps = sb;
-  return blocked;
+  return status;
 }
 
 // The empty-space phase for rays whose direction is positive on every axis
@@ -685,16 +751,19 @@ __device__ __forceinline__ bool dense_skip_empty(float Px, float Py, float Pz, f
 // iteration and takes the general branch (dense_step's).  Shadow rays start
 // on lattice points, so this correction is frequent for them: as a branch it
 // ran in about half of the wave-steps.
-template <int K>
-__device__ __forceinline__ bool dense_skip_positive(float dx, float dy, float dz, float ix,
-                                                    float iy, float iz, uint32_t mx, uint32_t my,
-                                                    uint32_t mz, DenseCell& s, int& steps,
-                                                    int limit, __amdgpu_buffer_rsrc_t bytes) {
+// The free-map test is dense_skip_empty's, on octant 0's bitmap (`fm`).
+template <int K, bool kFree>
+__device__ __forceinline__ int dense_skip_positive(float dx, float dy, float dz, float ix,
+                                                   float iy, float iz, uint32_t mx, uint32_t my,
+                                                   uint32_t mz, DenseCell& s, int& steps,
+                                                   int limit, __amdgpu_buffer_rsrc_t bytes,
+                                                   __amdgpu_buffer_rsrc_t fm) {
   float bx = s.wx, by = s.wy, bz = s.wz;
   uint32_t bc = s.c;
   int sb = steps;
-  bool blocked = false;
+  int status = kSkipLimit;
   while (sb + K < limit) {
+    const uint32_t fw = kFree ? free_word(fm, 0u, bc) : 0u;
     float wx = bx, wy = by, wz = bz;
     uint32_t c = bc;
     uint32_t any = 0u;
@@ -737,8 +806,12 @@ __device__ __forceinline__ bool dense_skip_positive(float dx, float dy, float dz
       }
       any |= (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(bytes, (int)a, 0, 0);
     }
+    if (kFree && free_bit(fw, bc)) {
+      status = kSkipFree;
+      break;
+    }
     if (stop | (any != 0u)) {
-      blocked = true;
+      status = kSkipBlocked;
       break;
     }
     bx = wx;
@@ -752,7 +825,7 @@ __device__ __forceinline__ bool dense_skip_positive(float dx, float dy, float dz
   s.wz = bz;
   s.c = bc;
   steps = sb;
-  return blocked;
+  return status;
 }
 
 // The flat lookup's traversal from r's block-level state: the empty-space
@@ -772,12 +845,20 @@ __device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const Den
   const __amdgpu_buffer_rsrc_t vox =
       __builtin_amdgcn_make_buffer_rsrc((void*)w.vox, (short)0, (int)w.nvox, 0x00020000);
   int steps0 = r.steps;
+  const __amdgpu_buffer_rsrc_t fm = free_rsrc(w.freemap, w.free_bytes);
+  int status;
   if constexpr (kPositive)
-    dense_skip_positive<VT_SKIP_STEPS>(r.dx, r.dy, r.dz, r.ix, r.iy, r.iz, mx, my, mz, s0, steps0,
-                                       max_steps, vox);
+    status = dense_skip_positive<VT_SKIP_STEPS, L::kFree>(r.dx, r.dy, r.dz, r.ix, r.iy, r.iz, mx,
+                                                          my, mz, s0, steps0, max_steps, vox, fm);
   else
-    dense_skip_empty<VT_SKIP_STEPS>(r.Px, r.Py, r.Pz, r.Ax, r.Ay, r.Az, r.dx, r.dy, r.dz, r.ix,
-                                    r.iy, r.iz, mx, my, mz, flip, s0, steps0, max_steps, vox);
+    status = dense_skip_empty<VT_SKIP_STEPS, L::kFree>(
+        r.Px, r.Py, r.Pz, r.Ax, r.Ay, r.Az, r.dx, r.dy, r.dz, r.ix, r.iy, r.iz, mx, my, mz, flip,
+        s0, steps0, max_steps, vox, fm, free_octant_offset(flip, mx, my, mz, w.free_oct_bytes));
+  if (L::kFree && status == kSkipFree) {  // a miss: no later lookup can be nonzero
+    h.hit = false;
+    h.steps = steps0 + VT_SKIP_STEPS;  // the steps executed, the last iteration's included
+    return;
+  }
   float fwx, fwy, fwz;
   uint32_t fc, fM, fa, fv;
   int sF;
@@ -810,78 +891,12 @@ __device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const Den
   h.steps = sF + 1;
 }
 
-// dense_run_prefetch with the empty-space phase limited to seg_limit: returns
-// false (r then holds the state to continue from: within, code, steps --
-// an iteration start whose lookups were all empty) when that phase ran out
-// of its limit before the ray stopped and before the ray's cap; true with h
-// filled otherwise.  A second call from that state with a limit at or above
-// the cap takes the same iterations as one uncut call.
-template <class L, bool kPositive = false>
-__device__ __forceinline__ bool dense_run_prefetch_seg(DenseRay& r, L& lk, const DenseWorld& w,
-                                                       DenseHit& h, int seg_limit) {
-  const uint32_t mx = in_vgpr(w.mx), my = in_vgpr(w.my), mz = in_vgpr(w.mz), flip = r.flip;
-  const int max_steps = r.max_steps;
-  const int limit = seg_limit < max_steps ? seg_limit : max_steps;
-  DenseCell s0;
-  s0.wx = r.wx;
-  s0.wy = r.wy;
-  s0.wz = r.wz;
-  s0.c = r.c;
-  const __amdgpu_buffer_rsrc_t vox =
-      __builtin_amdgcn_make_buffer_rsrc((void*)w.vox, (short)0, (int)w.nvox, 0x00020000);
-  int steps0 = r.steps;
-  bool blocked;
-  if constexpr (kPositive)
-    blocked = dense_skip_positive<VT_SKIP_STEPS>(r.dx, r.dy, r.dz, r.ix, r.iy, r.iz, mx, my, mz,
-                                                 s0, steps0, limit, vox);
-  else
-    blocked = dense_skip_empty<VT_SKIP_STEPS>(r.Px, r.Py, r.Pz, r.Ax, r.Ay, r.Az, r.dx, r.dy,
-                                              r.dz, r.ix, r.iy, r.iz, mx, my, mz, flip, s0,
-                                              steps0, limit, vox);
-  if (!blocked && limit < max_steps) {
-    r.wx = s0.wx;
-    r.wy = s0.wy;
-    r.wz = s0.wz;
-    r.c = s0.c;
-    r.steps = steps0;
-    return false;
-  }
-  float fwx, fwy, fwz;
-  uint32_t fc, fM, fa, fv;
-  int sF;
-  bool wrappedF;
-  dense_loop_prefetch(r.Px, r.Py, r.Pz, r.Ax, r.Ay, r.Az, r.dx, r.dy, r.dz, r.ix, r.iy, r.iz, mx,
-                      my, mz, flip, s0, steps0, max_steps, vox, fwx, fwy, fwz, fc, fM, fa, fv,
-                      sF, wrappedF);
-  (void)lk;
-  if (sF >= max_steps || wrappedF) {
-    h.hit = false;
-    h.steps = sF;
-    return true;
-  }
-  const uint32_t g = fc ^ flip;
-  const bool fx = fM == mx, fy = fM == my;
-  const int pm = (flip & fM) ? 0 : 1;
-  h.hit = true;
-  h.face = (fx ? 2 : (fy ? 4 : 6)) - pm;
-  h.bx = (int)compact1by2(fa);
-  h.by = (int)compact1by2(fa >> 1);
-  h.bz = (int)compact1by2(fa >> 2);
-  h.code = fa;
-  h.v = fv;
-  h.sub = (uint32_t)((int)fwx & 7) | ((uint32_t)((int)fwy & 7) << 3) |
-          ((uint32_t)((int)fwz & 7) << 6);
-  h.hx = (float)(compact1by2(g) << 3) + fwx;
-  h.hy = (float)(compact1by2(g >> 1) << 3) + fwy;
-  h.hz = (float)(compact1by2(g >> 2) << 3) + fwz;
-  h.steps = sF + 1;
-  return true;
-}
-
+// kStats with the free-region lookup keeps the fast loops: its statistics
+// count the steps executed, rays ending early included.
 template <bool kStats, class L>
 __device__ __forceinline__ void dense_run(DenseRay& r, L& lk, const DenseWorld& w, DenseHit& h,
                                           uint32_t& l1f, uint32_t& l0f) {
-  if constexpr (L::kPrefetch && !kStats) {
+  if constexpr (L::kPrefetch && (!kStats || L::kFree)) {
     dense_run_prefetch(r, lk, w, h);
     return;
   }
@@ -959,6 +974,9 @@ struct BrickmapWorld {
   const uint8_t* occ;
   uint32_t nvox;           // dim^3
   uint32_t mx, my, mz;     // dilated Morton masks of x, y, z
+  // free-region map of the occupancy bytes; free_bytes 0 = none
+  const uint32_t* freemap;
+  uint32_t free_bytes, free_oct_bytes;
 };
 
 // map_getVoxel (map.glsl:38-47) of block coordinates b; counts the chunk-table
@@ -1095,23 +1113,11 @@ __device__ bool trace_general(const BrickmapWorld& w, float ox, float oy, float 
 // kPositive: the caller guarantees a direction with no negative component (the
 // zero substitution below makes it positive), and the empty-space steps take
 // dense_skip_positive.
-// A ray's state at the start of an empty-space iteration (block level, every
-// lookup so far empty): the point where a shadow ray can be parked and
-// resumed with the same steps.
-struct BrickPark {
-  float wx, wy, wz;
-  uint32_t c;  // flipped Morton code of gridsCoords >> 3
-  int steps;
-};
-
-// kSeg: the first empty-space run stops after one iteration; a ray still
-// running then is parked in *park (h.steps = -1) instead of finished.
-// kResume: continue the ray parked in *from (ox, oy, oz are then unused).
-// Without either, the traversal compiles as it did before they existed.
-template <bool kPositive = false, bool kSeg = false, bool kResume = false>
+// kFree: the empty-space loop tests the free-region map of the occupancy
+// bytes and ends a ray that can hit nothing more.
+template <bool kPositive = false, bool kFree = false>
 __device__ bool trace_brickmap_fast(const BrickmapWorld& w, float ox, float oy, float oz, float dx,
-                                    float dy, float dz, int max_steps, GeneralHit& h,
-                                    BrickPark* park = nullptr, const BrickPark* from = nullptr) {
+                                    float dy, float dz, int max_steps, GeneralHit& h) {
   if (dx == 0.0f) dx = 0.001f;
   if (dy == 0.0f) dy = 0.001f;
   if (dz == 0.0f) dz = 0.001f;
@@ -1137,90 +1143,71 @@ __device__ bool trace_brickmap_fast(const BrickmapWorld& w, float ox, float oy, 
   h.hx = h.hy = h.hz = -1.0f;
   int step = 0;
   bool hit = false;
-  bool resume = false;
-  if constexpr (kResume) {
-    const uint32_t gc0 = from->c ^ flip;
-    g[0] = (int)(compact1by2(gc0) << 3);
-    g[1] = (int)(compact1by2(gc0 >> 1) << 3);
-    g[2] = (int)(compact1by2(gc0 >> 2) << 3);
-    wc[0] = from->wx;
-    wc[1] = from->wy;
-    wc[2] = from->wz;
-    s = 3;
-    step = from->steps;
-    resume = true;
-  }
-  bool first_run = true;
   for (;;) {
     // The literal loop (both levels) until the lane stands at an empty
     // block at the block level, hits, leaves the map or reaches the cap.
     // Lanes at an empty block wait here for the rest of the wave, so the
     // wave's empty-space runs below start together.
     bool fast = false;
-    if (kResume && resume) {  // straight to the empty-space run
-      fast = true;
-      resume = false;
-    } else {
-      for (;;) {
-        if (step >= max_steps || g[0] >= B || g[1] >= B || g[2] >= B || g[0] < 0 || g[1] < 0 ||
-            g[2] < 0)
-          break;
-        int p[3] = {g[0] + (int)wc[0], g[1] + (int)wc[1], g[2] + (int)wc[2]};
-        int b[3] = {p[0] >> 3, p[1] >> 3, p[2] >> 3};
-        uint32_t block = bm_block(w, b[0], b[1], b[2]);
-        if (block != 0u) {
-          uint32_t sub = bm_sub(w, block, p[0] % 8, p[1] % 8, p[2] % 8);
-          if (sub != 0u) {
-            h.data = sub;
-            h.face = m == 0 ? 2 - pv[0] : (m == 1 ? 4 - pv[1] : 6 - pv[2]);
-            h.bx = b[0];
-            h.by = b[1];
-            h.bz = b[2];
-            h.sub = (uint32_t)(p[0] & 7) | ((uint32_t)(p[1] & 7) << 3) | ((uint32_t)(p[2] & 7) << 6);
-            h.hx = (float)g[0] + wc[0];
-            h.hy = (float)g[1] + wc[1];
-            h.hz = (float)g[2] + wc[2];
-            h.steps = step + 1;
-            hit = true;
-            break;
-          }
-          if (s != 0) {
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-              g[i] += (int)wc[i];
-              wc[i] = wc[i] - floorf(wc[i]);
-            }
-            s = 0;
-          }
-        } else {
-          if (s != 3) {
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-              int lo = g[i] & 7;
-              wc[i] = wc[i] + (float)lo;
-              g[i] -= lo;
-            }
-            s = 3;
-          }
-          fast = true;  // this iteration's step onwards run in the loop below
+    for (;;) {
+      if (step >= max_steps || g[0] >= B || g[1] >= B || g[2] >= B || g[0] < 0 || g[1] < 0 ||
+          g[2] < 0)
+        break;
+      int p[3] = {g[0] + (int)wc[0], g[1] + (int)wc[1], g[2] + (int)wc[2]};
+      int b[3] = {p[0] >> 3, p[1] >> 3, p[2] >> 3};
+      uint32_t block = bm_block(w, b[0], b[1], b[2]);
+      if (block != 0u) {
+        uint32_t sub = bm_sub(w, block, p[0] % 8, p[1] % 8, p[2] % 8);
+        if (sub != 0u) {
+          h.data = sub;
+          h.face = m == 0 ? 2 - pv[0] : (m == 1 ? 4 - pv[1] : 6 - pv[2]);
+          h.bx = b[0];
+          h.by = b[1];
+          h.bz = b[2];
+          h.sub = (uint32_t)(p[0] & 7) | ((uint32_t)(p[1] & 7) << 3) | ((uint32_t)(p[2] & 7) << 6);
+          h.hx = (float)g[0] + wc[0];
+          h.hy = (float)g[1] + wc[1];
+          h.hz = (float)g[2] + wc[2];
+          h.steps = step + 1;
+          hit = true;
           break;
         }
-        float t[3];
+        if (s != 0) {
 #pragma unroll
-        for (int i = 0; i < 3; i++) t[i] = ((float)(pv[i] << s) - wc[i]) * inv[i];
-        m = t[0] < t[1] ? (t[0] < t[2] ? 0 : 2) : (t[1] < t[2] ? 1 : 2);
-        float tm = m == 0 ? t[0] : (m == 1 ? t[1] : t[2]);
-        int sm = m == 0 ? sgn[0] : (m == 1 ? sgn[1] : sgn[2]);
-        int pm = m == 0 ? pv[0] : (m == 1 ? pv[1] : pv[2]);
-        float am = (float)((1 - pm) << s) * 0.999f;
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-          float nw = wc[i] + d[i] * tm;
-          wc[i] = (i == m) ? am : nw;
-          g[i] += (i == m) ? sm * (1 << s) : 0;
+          for (int i = 0; i < 3; i++) {
+            g[i] += (int)wc[i];
+            wc[i] = wc[i] - floorf(wc[i]);
+          }
+          s = 0;
         }
-        step++;
+      } else {
+        if (s != 3) {
+#pragma unroll
+          for (int i = 0; i < 3; i++) {
+            int lo = g[i] & 7;
+            wc[i] = wc[i] + (float)lo;
+            g[i] -= lo;
+          }
+          s = 3;
+        }
+        fast = true;  // this iteration's step onwards run in the loop below
+        break;
       }
+      float t[3];
+#pragma unroll
+      for (int i = 0; i < 3; i++) t[i] = ((float)(pv[i] << s) - wc[i]) * inv[i];
+      m = t[0] < t[1] ? (t[0] < t[2] ? 0 : 2) : (t[1] < t[2] ? 1 : 2);
+      float tm = m == 0 ? t[0] : (m == 1 ? t[1] : t[2]);
+      int sm = m == 0 ? sgn[0] : (m == 1 ? sgn[1] : sgn[2]);
+      int pm = m == 0 ? pv[0] : (m == 1 ? pv[1] : pv[2]);
+      float am = (float)((1 - pm) << s) * 0.999f;
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        float nw = wc[i] + d[i] * tm;
+        wc[i] = (i == m) ? am : nw;
+        g[i] += (i == m) ? sm * (1 << s) : 0;
+      }
+      step++;
     }
     if (!fast) break;
     // empty block at the block level: the fast loop takes the next steps
@@ -1231,29 +1218,23 @@ __device__ bool trace_brickmap_fast(const BrickmapWorld& w, float ox, float oy, 
     s0.c = morton3((uint32_t)g[0] >> 3, (uint32_t)g[1] >> 3, (uint32_t)g[2] >> 3) ^ flip;
     const uint32_t mx = in_vgpr(w.mx), my = in_vgpr(w.my), mz = in_vgpr(w.mz);
     int s0steps = step;
-    int lim = max_steps;
-    if constexpr (kSeg) {
-      if (first_run && s0steps + VT_SKIP_STEPS + 1 < max_steps) lim = s0steps + VT_SKIP_STEPS + 1;
-    }
-    first_run = false;
-    bool blocked;
+    const int lim = max_steps;
+    // the free-region map of the occupancy bytes: block-level lookups only
+    // descend into nonempty blocks, so a ray whose block-level lookups all
+    // read empty from here on is a miss
+    const __amdgpu_buffer_rsrc_t fm = free_rsrc(w.freemap, w.free_bytes);
+    int status;
     if constexpr (kPositive)
-      blocked = dense_skip_positive<VT_SKIP_STEPS>(d[0], d[1], d[2], inv[0], inv[1], inv[2], mx,
-                                                   my, mz, s0, s0steps, lim, occ);
+      status = dense_skip_positive<VT_SKIP_STEPS, kFree>(d[0], d[1], d[2], inv[0], inv[1], inv[2],
+                                                         mx, my, mz, s0, s0steps, lim, occ, fm);
     else
-      blocked = dense_skip_empty<VT_SKIP_STEPS>(P3[0], P3[1], P3[2], A3[0], A3[1], A3[2], d[0],
-                                                d[1], d[2], inv[0], inv[1], inv[2], mx, my, mz,
-                                                flip, s0, s0steps, lim, occ);
-    if constexpr (kSeg) {
-      if (!blocked && lim < max_steps) {
-        park->wx = s0.wx;
-        park->wy = s0.wy;
-        park->wz = s0.wz;
-        park->c = s0.c;
-        park->steps = s0steps;
-        h.steps = -1;
-        return false;
-      }
+      status = dense_skip_empty<VT_SKIP_STEPS, kFree>(
+          P3[0], P3[1], P3[2], A3[0], A3[1], A3[2], d[0], d[1], d[2], inv[0], inv[1], inv[2], mx,
+          my, mz, flip, s0, s0steps, lim, occ, fm,
+          free_octant_offset(flip, mx, my, mz, w.free_oct_bytes));
+    if (kFree && status == kSkipFree) {
+      step = s0steps;
+      break;
     }
     float fw[3];
     uint32_t fc, fM, fa, fv;

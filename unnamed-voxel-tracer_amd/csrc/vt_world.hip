@@ -6,6 +6,9 @@
 // codes (a 4^3 brick, bit = code & 63), L1 = one uint64 per 4096 codes (a
 // 16^3 region, bit = brick index & 63 set when the brick has a solid voxel),
 // L1b = one uint64 per 32768 codes (a 32^3 region, bit per 8^3 brick).
+#include <algorithm>
+#include <vector>
+
 #include "vt_internal.hpp"
 
 namespace vt {
@@ -195,6 +198,174 @@ hipError_t launch_build_dense(const uint8_t* linear, uint32_t dim, uint8_t* vox,
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   return launch_build_l1b(l0, n0, nullptr, (uint32_t)l1b_words(dim), l1b, stream);
+}
+
+// ---- free-region map: where a ray's remaining path is provably empty ----
+// Regions are 16^3 blocks (one L1 word; R = max(dim / 16, 1) per axis).  For
+// each of the 8 direction octants o, free_o(r) = no region r' at or ahead of
+// r on every axis in o's travel direction (the forward orthant, r itself
+// included) holds a nonempty block.  Bit r (region Morton code) of octant
+// o's bitmap is free_o(r).  The traversal loops read it to end a ray that can
+// hit nothing more.
+
+// One thread per region (linear index rx + R (ry + R rz)); writes whether the
+// region holds a nonempty brick into all 8 octant planes of tmp.
+__global__ __launch_bounds__(256) void k_free_regions(const uint64_t* __restrict__ l1, uint32_t R,
+                                                      uint8_t* __restrict__ tmp) {
+  const uint32_t n = R * R * R;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t rx = i % R, ry = (i / R) % R, rz = i / (R * R);
+  const uint8_t v = l1[morton3(rx, ry, rz)] != 0ull ? 1 : 0;
+  for (uint32_t o = 0; o < 8; o++) tmp[(size_t)o * n + i] = v;
+}
+
+// One thread per (octant, line along `axis`): each region becomes the OR of
+// itself and every region ahead of it on that axis in the octant's travel
+// direction.  Three passes (x, y, z) give the forward-orthant OR.
+__global__ __launch_bounds__(256) void k_free_scan(uint8_t* __restrict__ tmp, uint32_t R,
+                                                   int axis) {
+  const uint32_t lines = R * R, n = lines * R;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 8u * lines) return;
+  const uint32_t o = t / lines, l = t % lines;
+  const uint32_t u = l % R, v = l / R;
+  size_t base, stride;
+  if (axis == 0) {
+    base = (size_t)R * (u + (size_t)R * v);
+    stride = 1;
+  } else if (axis == 1) {
+    base = u + (size_t)lines * v;
+    stride = R;
+  } else {
+    base = u + (size_t)R * v;
+    stride = lines;
+  }
+  uint8_t* p = tmp + (size_t)o * n + base;
+  uint8_t acc = 0;
+  if (((o >> axis) & 1u) == 0u) {  // positive: ahead = higher index
+    for (int k = (int)R - 1; k >= 0; --k) {
+      acc |= p[(size_t)k * stride];
+      p[(size_t)k * stride] = acc;
+    }
+  } else {
+    for (uint32_t k = 0; k < R; ++k) {
+      acc |= p[(size_t)k * stride];
+      p[(size_t)k * stride] = acc;
+    }
+  }
+}
+
+// One thread per (octant, bitmap word): bit j of word w = free_o(region with
+// Morton code 32 w + j).
+__global__ __launch_bounds__(256) void k_free_pack(const uint8_t* __restrict__ tmp, uint32_t R,
+                                                   uint32_t words, uint32_t* __restrict__ fm) {
+  const uint32_t n = R * R * R;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 8u * words) return;
+  const uint32_t o = t / words, wi = t % words;
+  uint32_t bits = 0;
+  for (uint32_t j = 0; j < 32; j++) {
+    const uint32_t rc = wi * 32u + j;
+    if (rc >= n) break;
+    const uint32_t x = compact1by2(rc), y = compact1by2(rc >> 1), z = compact1by2(rc >> 2);
+    if (!tmp[(size_t)o * n + x + (size_t)R * (y + (size_t)R * z)]) bits |= 1u << j;
+  }
+  fm[t] = bits;
+}
+
+uint32_t free_regions(uint32_t dim) { return dim >= 16u ? dim / 16u : 1u; }
+
+uint32_t free_words_per_octant(uint32_t dim) {
+  const uint32_t R = free_regions(dim);
+  return (R * R * R + 31u) / 32u;
+}
+
+hipError_t launch_build_freemap(const uint64_t* l1, uint32_t dim, uint8_t* tmp, uint32_t* fm,
+                                hipStream_t stream) {
+  const uint32_t R = free_regions(dim), n = R * R * R, words = free_words_per_octant(dim);
+  hipLaunchKernelGGL(k_free_regions, dim3((n + 255u) / 256u), dim3(256), 0, stream, l1, R, tmp);
+  hipError_t e = hipGetLastError();
+  for (int axis = 0; axis < 3 && e == hipSuccess; axis++) {
+    hipLaunchKernelGGL(k_free_scan, dim3((8u * R * R + 255u) / 256u), dim3(256), 0, stream, tmp,
+                       R, axis);
+    e = hipGetLastError();
+  }
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_free_pack, dim3((8u * words + 255u) / 256u), dim3(256), 0, stream, tmp, R,
+                     words, fm);
+  return hipGetLastError();
+}
+
+// Brick bits of a Morton-ordered byte array (the brickmap occupancy bytes):
+// one uint64 per 4096 codes, bit j set when bytes [64 (64 w + j), +64) hold
+// a nonzero byte.  One wave per word; an 8^3 map has one partial word.
+__global__ __launch_bounds__(256) void k_brick_bits(const uint8_t* __restrict__ bytes,
+                                                    uint64_t nbytes, uint32_t nwords,
+                                                    uint64_t* __restrict__ l1) {
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t lane = threadIdx.x & 63u;
+  if (wave >= nwords) return;  // whole waves leave together
+  const uint64_t first = ((uint64_t)wave * 64u + lane) * 64u;
+  bool any = false;
+  if (first < nbytes) {
+    const uint4* s = reinterpret_cast<const uint4*>(bytes + first);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint4 v = s[q];
+      any |= (v.x | v.y | v.z | v.w) != 0u;
+    }
+  }
+  const uint64_t m = __ballot(any);
+  if (lane == 0) l1[wave] = m;
+}
+
+uint32_t brick_bit_words(uint32_t dim) {
+  const uint64_t n = (uint64_t)dim * dim * dim;
+  return n >= 4096u ? (uint32_t)(n / 4096u) : 1u;
+}
+
+hipError_t launch_brick_bits(const uint8_t* bytes, uint32_t dim, uint64_t* l1,
+                             hipStream_t stream) {
+  const uint32_t nwords = brick_bit_words(dim);
+  hipLaunchKernelGGL(k_brick_bits, dim3((nwords + 3u) / 4u), dim3(256), 0, stream, bytes,
+                     (uint64_t)dim * dim * dim, nwords, l1);
+  return hipGetLastError();
+}
+
+hipError_t rebuild_freemap(vt_world* w, hipStream_t stream) {
+  w->free_valid = false;
+  if (!w->freemap) return hipSuccess;
+  if (w->kind == WorldKind::Dense) return launch_build_freemap(w->l1, w->dim, w->free_tmp,
+                                                               w->freemap, stream);
+  hipError_t e = launch_brick_bits(w->occ, w->dim, w->brick_bits, stream);
+  if (e != hipSuccess) return e;
+  return launch_build_freemap(w->brick_bits, w->dim, w->free_tmp, w->freemap, stream);
+}
+
+hipError_t finish_freemap(vt_world* w) {
+  w->free_valid = false;
+  w->free_frac = 0.0f;
+  if (!w->freemap) return hipSuccess;
+  const uint32_t words = free_words_per_octant(w->dim), R = free_regions(w->dim);
+  std::vector<uint32_t> h(8u * (size_t)words);
+  const hipError_t e = hipMemcpy(h.data(), w->freemap, h.size() * 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return e;
+  for (uint32_t o = 0; o < 8; o++) {
+    uint64_t n = 0;
+    for (uint32_t i = 0; i < words; i++) n += (uint64_t)__builtin_popcount(h[o * words + i]);
+    w->free_frac = std::max(w->free_frac, (float)((double)n / ((double)R * R * R)));
+  }
+  w->free_valid = true;
+  return hipSuccess;
+}
+
+bool use_free_map(const vt_world* w, const ShadeConsts& sc) {
+  if (!w->freemap || !w->free_valid || w->free_frac <= 0.0f) return false;
+  if (sc.flags & VT_FLAG_FREE_MAP_OFF) return false;
+  if (sc.flags & VT_FLAG_FREE_MAP_ON) return true;
+  return (sc.max_primary >= VT_FREE_MIN_CAP || sc.max_shadow >= VT_FREE_MIN_CAP) &&
+         w->free_frac * VT_FREE_MIN_SHARE >= 1.0f;
 }
 
 // One thread per block of the layers y < y_end of a brickmap world (linear

@@ -37,6 +37,10 @@ VT_FLAG_LAMBERT = 1 << 1
 VT_FLAG_ALBEDO_COMPACT = 1 << 2
 VT_FLAG_GRAD_L2 = 1 << 3
 VT_FLAG_PRESENT = 1 << 4
+VT_FLAG_FREE_MAP_ON = 1 << 5
+VT_FLAG_FREE_MAP_OFF = 1 << 6
+VT_FREE_MIN_CAP = 512
+VT_FREE_MIN_SHARE = 16
 VT_UPSTREAM_GRADIENT = 0
 VT_UPSTREAM_L2_TARGET = 1
 VT_TRAVERSAL_FLAT = 0
@@ -126,6 +130,8 @@ def lib() -> C.CDLL:
         "vt_world_info": (i32, [vp, C.POINTER(u32), C.POINTER(C.c_size_t)]),
         "vt_render": (i32, [vp, vp, C.POINTER(Camera), C.POINTER(RenderParams),
                             C.POINTER(Frame), vp]),
+        "vt_free_map_used": (i32, [vp, C.POINTER(RenderParams), C.POINTER(Frame),
+                                   C.POINTER(i32)]),
         "vt_present": (i32, [vp, vp, u32, u32, vp, vp, vp]),
         "vt_vox_info": (i32, [vp, C.c_size_t, C.POINTER(u32), C.POINTER(u32)]),
         "vt_atlas_load_vox": (i32, [vp, C.c_size_t, vp, C.POINTER(u32)]),

@@ -428,6 +428,15 @@ def _stream(stream):
     return current_stream() if stream is None else stream
 
 
+def free_map_used(world: World, frame: Frame, prm: N.RenderParams) -> bool:
+    """Whether render() with these arguments ends rays early in free regions
+    (vt_free_map_used)."""
+    used = C.c_int32(0)
+    N.check(N.lib().vt_free_map_used(world.h, C.byref(prm), C.byref(frame.struct()),
+                                     C.byref(used)), "vt_free_map_used")
+    return bool(used.value)
+
+
 def render(ctx: Context, world: World, cam: N.Camera, frame: Frame, prm: N.RenderParams,
            stream=None) -> Frame:
     if (prm.width, prm.height) != (frame.width, frame.height):
