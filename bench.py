@@ -545,6 +545,19 @@ AMBIENT_DIFF = 0.2
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
+def build_provenance():
+    """Which libvt.so this run loaded and whether it is the build of the
+    sources beside it (the build record unnamed-voxel-tracer_amd/build.py
+    writes next to the library)."""
+    import build as B
+    p = B.provenance()
+    rec = p.get("record") or {}
+    return {"lib": p["lib"], "built_from_current_sources": p["built_from_current_sources"],
+            "sources_sha256_16": (rec.get("sources_sha256") or "")[:16] or None,
+            "libvt_sha256_16": (rec.get("libvt_sha256") or "")[:16] or None,
+            "hipcc": rec.get("hipcc")}
+
+
 def measured_traffic(key):
     """HBM bytes per launch of a kernel/workload as measured with rocprofv3
     FETCH_SIZE and WRITE_SIZE passes, scaled by the copy-kernel calibration
@@ -922,6 +935,7 @@ def main():
         "cpu_baseline": cpu,
         "parity": parity,
         "fwd_bwd": fwd_bwd,
+        "build": build_provenance(),
     }
     print(json.dumps(line), flush=True)
     world.close()

@@ -105,3 +105,13 @@ int main(void) {
         C.sizeof(N.RenderParams), N.RenderParams.sun.offset, N.RenderParams.albedo.offset,
         C.sizeof(N.Frame)]
     assert N.Camera.view.offset == 16 and N.Camera.fov.offset == 80
+
+
+def test_library_is_the_build_of_these_sources():
+    """The build record beside libvt.so names this library and these sources
+    (rebuild after changing a source)."""
+    import build as B
+    p = B.provenance()
+    assert p["record"] is not None, "no build record: run unnamed-voxel-tracer_amd/build.py"
+    assert p["lib_sha256_matches"], "libvt.so differs from the build that wrote the record"
+    assert p["sources_sha256_matches"], "sources changed since libvt.so was built"

@@ -3,11 +3,16 @@
 libvt.so         HIP kernels + C-ABI (include/vt.h), gfx950 code objects.
 libvt_scenes.so  host C generators of the synthetic benchmark worlds.
 
-Both land in ``unnamed-voxel-tracer_amd/lib/``.  Run ``python build.py`` or
+Both land in ``unnamed-voxel-tracer_amd/lib/``, with ``libvt.build.json``:
+the SHA-256 of the sources and flags libvt.so was built from and of the
+library itself, so a run can tell whether the library it loads is the build
+of the sources beside it (:func:`provenance`).  Run ``python build.py`` or
 call :func:`build`.
 """
 from __future__ import annotations
 
+import hashlib
+import json
 import os
 import subprocess
 import sys
@@ -54,6 +59,50 @@ def _run(cmd, verbose):
         print(out, flush=True)
 
 
+def _sha256_file(path: str) -> str:
+    h = hashlib.sha256()
+    with open(path, "rb") as f:
+        for chunk in iter(lambda: f.read(1 << 20), b""):
+            h.update(chunk)
+    return h.hexdigest()
+
+
+def sources_digest() -> str:
+    """SHA-256 over libvt's sources (csrc/, include/vt.h) and its flags."""
+    h = hashlib.sha256()
+    files = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC))
+    files.append(os.path.join(ROOT, "include", "vt.h"))
+    for path in files:
+        h.update(os.path.relpath(path, ROOT).encode())
+        with open(path, "rb") as f:
+            h.update(f.read())
+    h.update(" ".join(HIP_FLAGS).encode())
+    return h.hexdigest()
+
+
+def provenance(lib_dir: str = None) -> dict:
+    """The build record of the libvt.so in lib_dir (default: the in-tree one)
+    checked against that library and today's sources:
+    built_from_current_sources is True when both match the record."""
+    lib_dir = lib_dir or LIB
+    lib = os.path.join(lib_dir, "libvt.so")
+    out = {"lib": os.path.relpath(lib, ROOT), "record": None,
+           "built_from_current_sources": False}
+    try:
+        with open(os.path.join(lib_dir, "libvt.build.json")) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return out
+    lib_sha = _sha256_file(lib) if os.path.exists(lib) else None
+    src_sha = sources_digest()
+    out.update(record={k: rec.get(k) for k in ("sources_sha256", "libvt_sha256", "hipcc")},
+               lib_sha256_matches=lib_sha == rec.get("libvt_sha256"),
+               sources_sha256_matches=src_sha == rec.get("sources_sha256"))
+    out["built_from_current_sources"] = bool(out["lib_sha256_matches"] and
+                                             out["sources_sha256_matches"])
+    return out
+
+
 def build(verbose: bool = True) -> None:
     """Compiles every HIP source to an object in parallel (one hipcc each),
     then links libvt.so; each translation unit holds its own kernels."""
@@ -74,8 +123,15 @@ def build(verbose: bool = True) -> None:
     with ThreadPoolExecutor(jobs) as ex:
         for f in [ex.submit(_run, c, verbose) for c in cmds]:
             f.result()
+    src_sha = sources_digest()
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o",
           os.path.join(LIB, "libvt.so"), *objs], verbose)
+    ver = subprocess.run([hipcc, "--version"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                         text=True).stdout.splitlines()
+    with open(os.path.join(LIB, "libvt.build.json"), "w") as f:
+        json.dump({"sources_sha256": src_sha,
+                   "libvt_sha256": _sha256_file(os.path.join(LIB, "libvt.so")),
+                   "hipcc": ver[0] if ver else None, "flags": HIP_FLAGS}, f, indent=1)
     _run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-ffp-contract=off", "-Wall",
           "-o", os.path.join(LIB, "libvt_scenes.so"),
           os.path.join(PKG, "scenes", "vt_scenes.c"), "-lm", "-lpthread"], verbose)

@@ -31,12 +31,14 @@ __global__ void k_batch_reset(uint32_t* __restrict__ queues) {
   if (threadIdx.x < kQueues) queues[threadIdx.x] = 0u;
 }
 
-// 8 waves per SIMD: the view record and queue state push the kernel to ~94
-// VGPRs (5 waves); capping it at 64 spills a few values that live outside the
-// traversal loops and hides the gathers' latency far better (C2: 19.5 ->
-// 16.3 ms per 64-view batch).
+// 7 waves per SIMD (72 VGPRs): the view record and queue state push the
+// kernel to ~94 VGPRs (5 waves); capping it spills values that live outside
+// the traversal loops and hides the gathers' latency far better (C2: 19.5 ->
+// 16.3 ms per 64-view batch at 8 waves).  With the free-region test seven
+// waves measure 3.81 / 3.84 ms against 4.01 / 3.91 at eight (fewer spills:
+// 132 vs 176 B per lane).
 #ifndef VT_BATCH_WAVES_PER_SIMD
-#define VT_BATCH_WAVES_PER_SIMD 8
+#define VT_BATCH_WAVES_PER_SIMD 7
 #endif
 template <class L, bool kStats, int kShade>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VT_BATCH_WAVES_PER_SIMD))) void k_render_batch(DenseWorld w, ShadeConsts sc,

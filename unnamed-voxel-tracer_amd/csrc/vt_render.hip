@@ -23,13 +23,20 @@
 
 namespace vt {
 
-// amdgpu_waves_per_eu: the reference composite at most 64 VGPRs (eight waves
-// per SIMD hide the gathers of the traversal loop; the spills it costs lie
-// outside it; 1080p 0.785 vs 0.804 ms at seven), the Lambert modes at most
-// 72 (seven waves: C3's Lambert forward 0.117 vs 0.120 ms per view).
+// amdgpu_waves_per_eu: at most 72 VGPRs (seven waves per SIMD) for every
+// shading mode.  The reference composite needs no scratch at 72 (56 B per
+// lane of spills at 64 with the free-region test, ~100 MB of spill stores
+// per 1080p frame); C1 at 1080p measures the same at eight, seven and six
+// waves (0.2785 / 0.2791 / 0.2789 ms), C3's 512x512 views 2% faster at
+// seven (0.1113 vs 0.1137 ms per view).  Before the free-region test eight
+// waves were faster (1080p 0.785 vs 0.804 ms): the traversal loop then held
+// most of the time and its gathers needed the waves.
+#ifndef VT_REF_WAVES_PER_SIMD
+#define VT_REF_WAVES_PER_SIMD 7
+#endif
 template <class L, bool kStats, int kShade>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(kShade == kShadeRef ? 8 : 7))) void k_render_dense(
+__attribute__((amdgpu_waves_per_eu(kShade == kShadeRef ? VT_REF_WAVES_PER_SIMD : 7))) void k_render_dense(
     DenseWorld w, ViewConsts v, ShadeConsts sc, FrameOut out, uint32_t log2_tw,
     uint32_t blocks_x) {
   const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
