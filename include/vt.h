@@ -252,6 +252,18 @@ int vt_world_get_voxels(vt_ctx* ctx, const vt_world* world, const uint32_t* xyz,
  * block.  chunks or pool may be NULL; *pool_words is always set; a
  * pool_capacity_words below the allocated words is an error
  * (VT_ERR_INVALID_ARGUMENT).  Returns when the outputs are written.       */
+/* The free-region map of a world (VT_FLAG_FREE_MAP_ON above): 8 octant
+ * bitmaps of *words_per_octant uint32 words each, octant o = (x negative) |
+ * (y negative) << 1 | (z negative) << 2, bit r = region with Morton code r
+ * (regions of 16^3 blocks, max(dim/16, 1) per axis) free for that octant.
+ * words may be NULL (only *words_per_octant is set); otherwise it holds
+ * 8 * *words_per_octant words.  *valid = 1 when the map is current (0 for
+ * worlds without one, or after a failed rebuild).  No counterpart in the
+ * reference; for checking the map against the world.  Returns when the
+ * words are written.                                                       */
+int vt_world_read_free_map(vt_ctx* ctx, const vt_world* world, uint32_t* words,
+                           size_t capacity_words, uint32_t* words_per_octant, int* valid,
+                           void* stream);
 int vt_world_read_brickmap(vt_ctx* ctx, const vt_world* world, uint32_t* chunks,
                            uint32_t* pool, size_t pool_capacity_words, size_t* pool_words,
                            void* stream);

@@ -186,6 +186,145 @@ def test_on_off_auto_agree_c1_row_band(ctx):
     w.close()
 
 
+def _morton3(x, y, z):
+    def part(t):
+        t = np.asarray(t, np.uint64) & 0x3FF
+        t = (t | (t << 16)) & 0xFF0000FF
+        t = (t | (t << 8)) & 0x0300F00F
+        t = (t | (t << 4)) & 0x030C30C3
+        return (t | (t << 2)) & 0x09249249
+    return part(x) | (part(y) << 1) | (part(z) << 2)
+
+
+def _region_occupancy(grid):
+    """[z, y, x] regions of 16^3 blocks (one region for smaller grids) that
+    hold a nonzero block."""
+    n = grid.shape[0]
+    if n < 16:
+        return np.array([[[grid.any()]]])
+    R = n // 16
+    return grid.reshape(R, 16, R, 16, R, 16).any(axis=(1, 3, 5))
+
+
+def _pack(free):
+    """Octant bitmaps as the library lays them out: bit r = region with
+    Morton code r (free is [8, z, y, x] booleans)."""
+    R = free.shape[1]
+    z, y, x = np.meshgrid(np.arange(R), np.arange(R), np.arange(R), indexing="ij")
+    codes = _morton3(x.ravel(), y.ravel(), z.ravel()).astype(np.int64)
+    words = max(1, (R ** 3 + 31) // 32)
+    out = np.zeros((8, words), np.uint32)
+    for o in range(8):
+        bits = free[o].ravel()
+        for c, b in zip(codes, bits):
+            if b:
+                out[o, c >> 5] |= np.uint32(1 << (c & 31))
+    return out
+
+
+def expected_free_brute(grid):
+    """The definition, region pair by region pair: region r is free for
+    octant o when no occupied region r' lies at or ahead of r on every axis
+    in o's travel direction."""
+    occ = _region_occupancy(grid)
+    R = occ.shape[0]
+    occupied = np.argwhere(occ)  # [z, y, x]
+    free = np.ones((8, R, R, R), bool)
+    for o in range(8):
+        sx, sy, sz = (-1 if o & 1 else 1), (-1 if o & 2 else 1), (-1 if o & 4 else 1)
+        for z in range(R):
+            for y in range(R):
+                for x in range(R):
+                    for (oz, oy, ox) in occupied:
+                        if (ox - x) * sx >= 0 and (oy - y) * sy >= 0 and (oz - z) * sz >= 0:
+                            free[o, z, y, x] = False
+                            break
+    return _pack(free)
+
+
+def expected_free_scan(grid):
+    """The same map from running ORs of the region occupancy along each axis
+    in the travel direction (for grids too large for the pairwise form)."""
+    occ = _region_occupancy(grid)
+    free = np.zeros((8,) + occ.shape, bool)
+    for o in range(8):
+        a = occ.copy()
+        for axis, bit in ((2, 0), (1, 1), (0, 2)):
+            if (o >> bit) & 1:
+                a = np.maximum.accumulate(a, axis=axis)
+            else:
+                a = np.flip(np.maximum.accumulate(np.flip(a, axis), axis=axis), axis)
+        free[o] = ~a
+    return _pack(free)
+
+
+@pytest.mark.parametrize("n", [16, 32, 64, 128])
+def test_map_equals_its_definition(ctx, n):
+    """The device-built map, bit for bit, against the definition evaluated
+    region pair by region pair, on the dense layout and (same occupancy) the
+    reference layout, then again after edits that add and clear blocks."""
+    import brickmap_util as BU
+    pal = S.palette()
+    grid = boxes_world(n, 500 + n, n_boxes=3, n_single=4)
+    want = expected_free_brute(grid)
+    wd = vt.World.dense(ctx, grid, pal)
+    got, valid = wd.read_free_map()
+    assert valid and np.array_equal(got, want), f"dense n={n}"
+    dim, chunks, pool, atlas = BU.embedding_a(grid, pal)
+    wb = vt.World.brickmap(ctx, dim, chunks, pool, atlas)
+    got, valid = wb.read_free_map()
+    assert valid and np.array_equal(got, want), f"reference layout n={n}"
+    wb.close()
+    # edits: a voxel in a far corner region and the removal of a box's corner
+    rng = np.random.default_rng(n)
+    xyz = rng.integers(0, n, (6, 3)).astype(np.uint32)
+    vals = np.array([3, 0, 5, 0, 7, 2], np.uint32)
+    wd.set_voxels(xyz, vals)
+    for (x, y, z), v in zip(xyz, vals):
+        grid[z, y, x] = v
+    got, valid = wd.read_free_map()
+    assert valid and np.array_equal(got, expected_free_brute(grid)), f"dense n={n} after edits"
+    wd.close()
+
+
+def test_map_of_empty_and_solid_worlds(ctx):
+    """A world with nothing in it is free everywhere for every octant; one
+    with a block in every region nowhere; the smallest reference layout (8^3
+    blocks, one region) too."""
+    pal = S.palette()
+    for n, fill, want_bit in ((16, 0, 1), (64, 0, 1), (64, 9, 0)):
+        g = np.zeros((n, n, n), np.uint8)
+        if fill:
+            g[::16, ::16, ::16] = 2  # one block per region
+        w = vt.World.dense(ctx, g, pal)
+        got, valid = w.read_free_map()
+        R = max(n // 16, 1)
+        full = np.zeros(got.shape[1], np.uint32)
+        for r in range(R ** 3):
+            full[r >> 5] |= np.uint32((1 << (r & 31)) if want_bit else 0)
+        assert valid and all(np.array_equal(got[o], full) for o in range(8)), (n, fill)
+        w.close()
+    atlas = np.zeros((256, 256, 256), np.uint32)
+    w = vt.World.brickmap(ctx, 8, np.zeros(1, np.uint32), np.zeros(0, np.uint32), atlas)
+    got, valid = w.read_free_map()
+    assert valid and got.shape == (8, 1) and np.all(got == 1)
+    w.close()
+
+
+def test_map_of_the_headline_world(ctx):
+    """The 512^3 headline world (32^3 regions): the device map against the
+    running-OR form; 84% of the regions free for up-going rays."""
+    grid = S.generate(S.spec("c1"))
+    w = vt.World.dense(ctx, grid, S.palette())
+    got, valid = w.read_free_map()
+    want = expected_free_scan(grid)
+    assert valid and np.array_equal(got, want)
+    share = [sum(bin(int(x)).count("1") for x in got[o]) / 32 ** 3 for o in range(8)]
+    log(f"c1 share of free regions per octant: {[round(x, 3) for x in share]}")
+    assert max(share) > 0.8
+    w.close()
+
+
 def test_statistics_of_executed_steps(ctx):
     """Statistics with the test forced on count the steps the kernel
     executed: fewer than traceMap's (rays end early), with the same hits,

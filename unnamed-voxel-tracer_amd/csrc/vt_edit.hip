@@ -475,6 +475,27 @@ int vt_world_read_brickmap(vt_ctx* ctx, const vt_world* world, uint32_t* chunks,
   return VT_OK;
 }
 
+int vt_world_read_free_map(vt_ctx* ctx, const vt_world* world, uint32_t* words,
+                           size_t capacity_words, uint32_t* words_per_octant, int* valid,
+                           void* stream) {
+  if (!ctx || !world || !words_per_octant || !valid)
+    return fail(VT_ERR_NULL_POINTER, "vt_world_read_free_map: null argument");
+  if (world->ctx != ctx) return fail(VT_ERR_WRONG_CONTEXT, "vt_world_read_free_map");
+  const uint32_t wpo = vt::free_words_per_octant(world->dim);
+  *words_per_octant = wpo;
+  *valid = world->freemap && world->free_valid ? 1 : 0;
+  if (!words || !world->freemap) return VT_OK;
+  if (capacity_words < 8u * (size_t)wpo)
+    return fail(VT_ERR_INVALID_ARGUMENT,
+                "vt_world_read_free_map: the map holds %u words, the output room is %zu",
+                8u * wpo, capacity_words);
+  VT_HIP(hipSetDevice(ctx->device));
+  const hipStream_t s = (hipStream_t)stream;
+  VT_HIP(hipMemcpyAsync(words, world->freemap, 8u * (size_t)wpo * 4u, hipMemcpyDeviceToHost, s));
+  VT_HIP(hipStreamSynchronize(s));
+  return VT_OK;
+}
+
 int vt_pick(vt_ctx* ctx, const vt_world* world, const vt_camera* camera, int32_t max_steps,
             vt_pick_result* result, void* stream) {
   if (!ctx || !world || !camera || !result)

@@ -132,6 +132,8 @@ def lib() -> C.CDLL:
                             C.POINTER(Frame), vp]),
         "vt_free_map_used": (i32, [vp, C.POINTER(RenderParams), C.POINTER(Frame),
                                    C.POINTER(i32)]),
+        "vt_world_read_free_map": (i32, [vp, vp, vp, C.c_size_t, C.POINTER(u32), C.POINTER(i32),
+                                         vp]),
         "vt_present": (i32, [vp, vp, u32, u32, vp, vp, vp]),
         "vt_vox_info": (i32, [vp, C.c_size_t, C.POINTER(u32), C.POINTER(u32)]),
         "vt_atlas_load_vox": (i32, [vp, C.c_size_t, vp, C.POINTER(u32)]),

@@ -189,6 +189,21 @@ class World:
                 "vt_world_read_brickmap")
         return chunks, pool
 
+    def read_free_map(self, stream=None):
+        """(bitmaps, valid): the free-region map as a uint32 array [8, words per
+        octant] (octant o = x negative | y negative << 1 | z negative << 2; bit r
+        = region with Morton code r free), None for a world without one."""
+        wpo, valid = C.c_uint32(), C.c_int32()
+        N.check(N.lib().vt_world_read_free_map(self.ctx.h, self.h, None, 0, C.byref(wpo),
+                                               C.byref(valid), C.c_void_p(_stream(stream))),
+                "vt_world_read_free_map")
+        words = np.zeros(8 * wpo.value, np.uint32)
+        N.check(N.lib().vt_world_read_free_map(self.ctx.h, self.h, words.ctypes.data, words.size,
+                                               C.byref(wpo), C.byref(valid),
+                                               C.c_void_p(_stream(stream))),
+                "vt_world_read_free_map")
+        return words.reshape(8, wpo.value), bool(valid.value)
+
     def procgen(self, offset_x: float = 0.0, offset_y: float = 0.0, stream=None):
         """procgen.zig:6-70 on this (empty) brickmap world; see vt_world_procgen."""
         N.check(N.lib().vt_world_procgen(self.ctx.h, self.h, offset_x, offset_y,
