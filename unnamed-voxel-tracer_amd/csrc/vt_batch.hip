@@ -200,6 +200,8 @@ hipError_t launch_batch(const vt_world* world, const vt::ShadeConsts& sc_in,
   } else {
     if (stats) VT_BATCH_SHADE(FlatLookup, true)
     else if (w.free_bytes) VT_BATCH_SHADE(FlatLookupFree, false)
+    else if (sc.max_primary >= vt::kLongCap || sc.max_shadow >= vt::kLongCap)
+      VT_BATCH_SHADE(FlatLookupLong, false)
     else VT_BATCH_SHADE(FlatLookup, false)
   }
 #undef VT_BATCH_SHADE

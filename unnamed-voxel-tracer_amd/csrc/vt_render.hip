@@ -186,11 +186,14 @@ hipError_t launch_render_dense(const DenseWorld& w_in, uint32_t traversal, const
     if (stats) launch_dense_shade<BitmaskLookup8, true>(w, v, sc, out, lw, blocks_x, grid, stream);
     else launch_dense_shade<BitmaskLookup8, false>(w, v, sc, out, lw, blocks_x, grid, stream);
   } else {
+    const bool long_caps = sc.max_primary >= kLongCap || sc.max_shadow >= kLongCap;
     if (stats && w.free_bytes)  // statistics of the steps executed
       launch_dense_shade<FlatLookupFree, true>(w, v, sc, out, lw, blocks_x, grid, stream);
     else if (stats) launch_dense_shade<FlatLookup, true>(w, v, sc, out, lw, blocks_x, grid, stream);
     else if (w.free_bytes)
       launch_dense_shade<FlatLookupFree, false>(w, v, sc, out, lw, blocks_x, grid, stream);
+    else if (long_caps)
+      launch_dense_shade<FlatLookupLong, false>(w, v, sc, out, lw, blocks_x, grid, stream);
     else launch_dense_shade<FlatLookup, false>(w, v, sc, out, lw, blocks_x, grid, stream);
   }
   return hipGetLastError();

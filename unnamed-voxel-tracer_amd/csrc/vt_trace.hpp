@@ -236,11 +236,23 @@ __device__ __forceinline__ bool in_dim(const DenseWorld& w, int bx, int by, int 
   return ((uint32_t)bx < w.dim) & ((uint32_t)by < w.dim) & ((uint32_t)bz < w.dim);
 }
 
+// Steps per iteration of the empty-space phase (dense_skip_empty): 6 for
+// frames with a step cap of at least 512, 5 below.  Measured on MI355X with
+// the free-region test: C1 at 1080p 0.2740 / 0.2725 / 0.2710 ms at K = 6 /
+// 7 / 8 against 0.2796 at 5; C3's views 0.1065 ms against 0.1117; the C1
+// world in the reference layout 0.2858 (6) / 0.2892 (7) / 0.292 (8) against
+// 0.2869 (5); the engine's own frame (caps 192/48, short runs between model
+// blocks) 0.2747 at 5 against 0.2972 at 6 and 0.335 at 7.
+constexpr int kSkipShort = 5;
+constexpr int kSkipLong = 6;
+constexpr int kLongCap = 512;
+
 // Occupancy probe strategies (identical answers by contract).
 struct FlatLookup {
   // one byte gather per probe; returns the material (0 = empty)
   static constexpr bool kPrefetch = true;  // probes have no state: run ahead
   static constexpr bool kFree = false;     // no free-region test
+  static constexpr int kSkip = kSkipShort;  // empty-space iteration length
   __device__ __forceinline__ void reset() {}
   template <bool kStats>
   __device__ __forceinline__ uint32_t probe(const DenseWorld& w, uint32_t code, uint32_t&,
@@ -268,6 +280,7 @@ struct BitmaskLookupT {
   static_assert(kBrick == 4 || kBrick == 8, "4^3 or 8^3 bricks");
   static constexpr bool kPrefetch = false;
   static constexpr bool kFree = false;
+  static constexpr int kSkip = kSkipShort;  // (the plain loop has no empty-space phase)
   static constexpr uint32_t kBrickShift = kBrick == 8 ? 9u : 6u;  // log2 codes per brick
   uint32_t brick, region, leaf;
   uint64_t bword, rword;
@@ -318,6 +331,13 @@ using BitmaskLookup8 = BitmaskLookupT<8>;
 // free_rsrc below) and end a ray that can hit nothing more.
 struct FlatLookupFree : FlatLookup {
   static constexpr bool kFree = true;
+  static constexpr int kSkip = kSkipLong;
+};
+
+// The flat lookup for frames with long step caps: the longer empty-space
+// iteration, no free-region test.
+struct FlatLookupLong : FlatLookup {
+  static constexpr int kSkip = kSkipLong;
 };
 
 // Result of one traceMap call in a dense world.
@@ -617,11 +637,6 @@ __device__ __forceinline__ void dense_loop_prefetch(
   fv = useB ? vB : vA;
 }
 
-// Steps per iteration of the empty-space phase below.
-#ifndef VT_SKIP_STEPS
-#define VT_SKIP_STEPS 5
-#endif
-
 // How an empty-space run ended.
 enum SkipStatus : int {
   kSkipLimit = 0,    // reached its step limit; every lookup so far empty
@@ -848,15 +863,15 @@ __device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const Den
   const __amdgpu_buffer_rsrc_t fm = free_rsrc(w.freemap, w.free_bytes);
   int status;
   if constexpr (kPositive)
-    status = dense_skip_positive<VT_SKIP_STEPS, L::kFree>(r.dx, r.dy, r.dz, r.ix, r.iy, r.iz, mx,
+    status = dense_skip_positive<L::kSkip, L::kFree>(r.dx, r.dy, r.dz, r.ix, r.iy, r.iz, mx,
                                                           my, mz, s0, steps0, max_steps, vox, fm);
   else
-    status = dense_skip_empty<VT_SKIP_STEPS, L::kFree>(
+    status = dense_skip_empty<L::kSkip, L::kFree>(
         r.Px, r.Py, r.Pz, r.Ax, r.Ay, r.Az, r.dx, r.dy, r.dz, r.ix, r.iy, r.iz, mx, my, mz, flip,
         s0, steps0, max_steps, vox, fm, free_octant_offset(flip, mx, my, mz, w.free_oct_bytes));
   if (L::kFree && status == kSkipFree) {  // a miss: no later lookup can be nonzero
     h.hit = false;
-    h.steps = steps0 + VT_SKIP_STEPS;  // the steps executed, the last iteration's included
+    h.steps = steps0 + L::kSkip;  // the steps executed, the last iteration's included
     return;
   }
   float fwx, fwy, fwz;
@@ -1114,10 +1129,12 @@ __device__ bool trace_general(const BrickmapWorld& w, float ox, float oy, float 
 // zero substitution below makes it positive), and the empty-space steps take
 // dense_skip_positive.
 // kFree: the empty-space loop tests the free-region map of the occupancy
-// bytes and ends a ray that can hit nothing more.
+// bytes and ends a ray that can hit nothing more; the launcher takes it for
+// long step caps, so it also takes the longer empty-space iteration.
 template <bool kPositive = false, bool kFree = false>
 __device__ bool trace_brickmap_fast(const BrickmapWorld& w, float ox, float oy, float oz, float dx,
                                     float dy, float dz, int max_steps, GeneralHit& h) {
+  constexpr int kSkip = kFree ? kSkipLong : kSkipShort;
   if (dx == 0.0f) dx = 0.001f;
   if (dy == 0.0f) dy = 0.001f;
   if (dz == 0.0f) dz = 0.001f;
@@ -1225,10 +1242,10 @@ __device__ bool trace_brickmap_fast(const BrickmapWorld& w, float ox, float oy, 
     const __amdgpu_buffer_rsrc_t fm = free_rsrc(w.freemap, w.free_bytes);
     int status;
     if constexpr (kPositive)
-      status = dense_skip_positive<VT_SKIP_STEPS, kFree>(d[0], d[1], d[2], inv[0], inv[1], inv[2],
+      status = dense_skip_positive<kSkip, kFree>(d[0], d[1], d[2], inv[0], inv[1], inv[2],
                                                          mx, my, mz, s0, s0steps, lim, occ, fm);
     else
-      status = dense_skip_empty<VT_SKIP_STEPS, kFree>(
+      status = dense_skip_empty<kSkip, kFree>(
           P3[0], P3[1], P3[2], A3[0], A3[1], A3[2], d[0], d[1], d[2], inv[0], inv[1], inv[2], mx,
           my, mz, flip, s0, s0steps, lim, occ, fm,
           free_octant_offset(flip, mx, my, mz, w.free_oct_bytes));
