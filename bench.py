@@ -881,6 +881,14 @@ def main():
     ms = float(np.mean(kms))
     print(f"[bench] {args.steps} steps, kernel mean {ms:.3f} ms, min {min(kms):.3f} ms",
           file=sys.stderr, flush=True)
+    # the same frame in the fixed launch order (a frame without a tile
+    # schedule), reported beside the timed figure
+    ff = vt.Frame.alloc(W, H, schedule=False)
+    fixed_ms = timed(torch, stream, lambda: vt.render(ctx, world, cam, ff, prm), args.steps,
+                     max(1, args.warmup))
+    print(f"[bench] fixed launch order: kernel mean {fixed_ms:.3f} ms", file=sys.stderr,
+          flush=True)
+    del ff
 
     npix = W * H
     rays = npix + stats["primary_hits"]
@@ -922,6 +930,9 @@ def main():
             "gsteps_per_s": round(steps_total / (ms * 1e-3) / 1e9, 2),
             "free_region_test": free,
             "executed_steps": executed,
+            "tile_schedule": "blocks launched costliest first, in the order the previous "
+                             "render into the frame measured (vt_frame.schedule)",
+            "kernel_ms_fixed_order": round(fixed_ms, 4),
         },
         "roofline": {
             "bound": "hbm",

@@ -178,7 +178,29 @@ typedef struct vt_frame {
   uint32_t* display;     /* W*H RGBA8 (R in the low byte): VT_FLAG_PRESENT    */
                          /* colour stored as an rgba8 framebuffer does        */
                          /* (clamp to [0,1], round to nearest)                */
+  uint32_t* schedule;    /* VT_SCHEDULE_WORDS(W, H) words, zero-filled before */
+                         /* its first render: the tile schedule (below)       */
 } vt_frame;
+
+/* Tile schedule (vt_frame.schedule; vt_render only, the batch ignores it).
+ * No counterpart in the reference, which dispatches its compute shaders over
+ * the image in a fixed order (game.zig:241-245).  vt_render launches a
+ * frame's 256-pixel blocks, more than the GPU holds at once; in image order
+ * the costly blocks (rays that skim the terrain toward the horizon: several
+ * times the steps of the others) start mid-launch and run on alone at its
+ * end.  With a schedule buffer each render measures the time of every block
+ * and the next render into the same frame launches the blocks most costly
+ * first, from that record.  The images are the same in either
+ * order: only which launch slot renders which block changes.  The buffer is
+ * private to one frame (W x H); zero-fill it before its first render (and
+ * again before using it for another image size: a buffer last used at another
+ * size is detected and restarted, but only a zero-filled one is guaranteed
+ * to be).  Renders into one frame must not overlap, as for its images.  The
+ * first render into a frame, and renders into frames without a schedule, use
+ * the fixed order. */
+#define VT_SCHEDULE_BLOCKS(W, H) ((((W) + 15u) / 16u) * (((H) + 7u) / 8u))
+#define VT_SCHEDULE_WORDS(W, H) \
+  (512u + 3u * (4096u + 128u * ((VT_SCHEDULE_BLOCKS(W, H) + 7u) / 8u)))
 
 /* ---- context ------------------------------------------------------------ */
 /* Replaces the GL context made current in gfx.init (graphics.zig:60-75). */
@@ -370,7 +392,8 @@ typedef struct vt_view {
   vt_camera camera;
   uint32_t width, height;
   uint32_t _pad[2];
-  vt_frame frame; /* this view's outputs; frame.stats is ignored */
+  vt_frame frame; /* this view's outputs; frame.stats and frame.schedule */
+                  /* are ignored                                        */
 } vt_view;
 
 /* wave tile shapes (log2 of the tile width; 64 pixels per tile) */

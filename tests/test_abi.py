@@ -91,20 +91,34 @@ int main(void) {
          offsetof(vt_camera, fov), sizeof(vt_hit_record));
   printf("%zu %zu %zu %zu\n", sizeof(vt_render_params), offsetof(vt_render_params, sun),
          offsetof(vt_render_params, albedo), sizeof(vt_frame));
+  printf("%zu %zu %zu\n", offsetof(vt_frame, schedule), sizeof(vt_view), offsetof(vt_view, frame));
+  const unsigned wh[][2] = {{1, 1}, {256, 256}, {1280, 720}, {1920, 1080}, {17, 1001}, {16384, 16384}};
+  for (int i = 0; i < 6; i++)
+    printf("%u %u %u ", (unsigned)VT_SCHEDULE_BLOCKS(wh[i][0], wh[i][1]),
+           (unsigned)VT_SCHEDULE_WORDS(wh[i][0], wh[i][1]), wh[i][0] * 100000u + wh[i][1]);
+  printf("\n");
   return 0;
 }
 ''')
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
                    check=True)
-    a, b = subprocess.run([str(exe)], capture_output=True, text=True,
-                          check=True).stdout.split("\n")[:2]
+    a, b, c, d = subprocess.run([str(exe)], capture_output=True, text=True,
+                                check=True).stdout.split("\n")[:4]
     # std140 u_Camera: vec4 @0, mat4 @16, float @80, 96 bytes (camera.glsl:2-6)
     assert [int(x) for x in a.split()] == [96, 16, 80, 12]
     assert [int(x) for x in b.split()] == [
         C.sizeof(N.RenderParams), N.RenderParams.sun.offset, N.RenderParams.albedo.offset,
         C.sizeof(N.Frame)]
     assert N.Camera.view.offset == 16 and N.Camera.fov.offset == 80
+    assert [int(x) for x in c.split()] == [N.Frame.schedule.offset, C.sizeof(N.View),
+                                           N.View.frame.offset]
+    vals = [int(x) for x in d.split()]
+    for i in range(0, len(vals), 3):
+        W, H = divmod(vals[i + 2], 100000)
+        assert vals[i] == N.schedule_blocks(W, H) and vals[i + 1] == N.schedule_words(W, H)
+        # the blocks of either tile shape vt_render uses fit the lists
+        assert N.schedule_blocks(W, H) >= max(-(-W // 32) * -(-H // 8), -(-W // 16) * -(-H // 16))
 
 
 def test_library_is_the_build_of_these_sources():

@@ -211,8 +211,9 @@ int check_frame(const vt_frame* frame, uint32_t flags, const char* who) {
     if (q && !aligned(q, 16)) return fail(VT_ERR_MISALIGNED, "%s: image not 16B aligned", who);
   if ((frame->hits && !aligned(frame->hits, 4)) || (frame->steps && !aligned(frame->steps, 4)) ||
       (frame->display && !aligned(frame->display, 4)) ||
+      (frame->schedule && !aligned(frame->schedule, 4)) ||
       (frame->stats && !aligned(frame->stats, 8)))
-    return fail(VT_ERR_MISALIGNED, "%s: hits/steps/stats misaligned", who);
+    return fail(VT_ERR_MISALIGNED, "%s: hits/steps/display/schedule/stats misaligned", who);
   return VT_OK;
 }
 
@@ -524,13 +525,13 @@ int vt_render(vt_ctx* ctx, const vt_world* world, const vt_camera* camera,
   if (world->kind == WorldKind::Dense) {
     vt::DenseWorld dw = vt::dense_view(world);
     if (!free_map) dw.free_bytes = 0;
-    e = vt::launch_render_dense(dw, p.traversal, v, sc, o, tile, (uint32_t)ctx->compute_units,
-                                (hipStream_t)stream);
+    e = vt::launch_render_dense(dw, p.traversal, v, sc, o, frame->schedule, tile,
+                                (uint32_t)ctx->compute_units, (hipStream_t)stream);
   } else {
     vt::BrickmapWorld bw = vt::brickmap_view(world);
     if (!free_map) bw.free_bytes = 0;
-    e = vt::launch_render_brickmap(bw, v, sc, o, tile, (uint32_t)ctx->compute_units,
-                                   (hipStream_t)stream);
+    e = vt::launch_render_brickmap(bw, v, sc, o, frame->schedule, tile,
+                                   (uint32_t)ctx->compute_units, (hipStream_t)stream);
   }
   if (e != hipSuccess) return hip_fail(e, "vt_render: launch");
   return VT_OK;

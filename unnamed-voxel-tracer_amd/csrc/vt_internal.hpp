@@ -122,12 +122,13 @@ ViewConsts view_consts(const vt_camera* camera, uint32_t W, uint32_t H);
 ShadeConsts shade_consts(const vt_render_params& p);
 FrameOut frame_out(const vt_frame* frame);
 
+// `schedule`: the frame's tile schedule buffer (vt_frame.schedule) or null.
 hipError_t launch_render_dense(const DenseWorld& w, uint32_t traversal, const ViewConsts& v,
-                               const ShadeConsts& sc, const FrameOut& out, TileShape tile,
-                               uint32_t compute_units, hipStream_t stream);
+                               const ShadeConsts& sc, const FrameOut& out, uint32_t* schedule,
+                               TileShape tile, uint32_t compute_units, hipStream_t stream);
 hipError_t launch_render_brickmap(const BrickmapWorld& w, const ViewConsts& v,
-                                  const ShadeConsts& sc, const FrameOut& out, TileShape tile,
-                                  uint32_t compute_units, hipStream_t stream);
+                                  const ShadeConsts& sc, const FrameOut& out, uint32_t* schedule,
+                                  TileShape tile, uint32_t compute_units, hipStream_t stream);
 
 // World construction kernels.
 // occ[morton(b)] = (map_getVoxel(b) != 0) for every block b of a brickmap

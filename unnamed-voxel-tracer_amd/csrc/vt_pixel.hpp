@@ -26,6 +26,209 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t n) {
   return ((j / kXcdStrip) * X + x) * kXcdStrip + j % kXcdStrip;
 }
 
+// ---- Tile schedule (vt_frame.schedule) ----
+// A frame's blocks are dispatched in launch order, round-robin over the 8
+// XCDs (launch slot i runs on XCD i % 8), as wave slots free up.  In a
+// terrain frame the blocks whose rays skim the terrain toward the horizon
+// cost many times the others (a 1080p frame of the 512^3 world: 100-170 us
+// per wave against 10-40 us elsewhere); in image order they start mid-launch
+// and run on alone at its end (the launch's waves then hold 71% of the wave
+// slots on average).  Launched most costly first, the cheap blocks fill in
+// around them.  A block's cost is known once it has run: its first wave
+// measures its time on the constant 100 MHz clock (the four waves of a block
+// trace neighbouring tiles: at 1080p the first wave's time correlates 0.999
+// with the block's longest), the block goes on its XCD's list of its log2
+// cost class, and the next render into the same frame launches each XCD's
+// blocks class by class from the costliest.  Each XCD keeps the image blocks
+// the fixed order gives it (xcd_remap's strips: the L2 locality of that
+// order; a single order over all XCDs made the engine's frame slower), and
+// only which of its launch slots renders which of them changes.
+//
+// Buffer (32-bit words; B = VT_SCHEDULE_BLOCKS(W, H) >= the blocks of either
+// tile shape, C = ceil(B / 8) >= the blocks of one XCD): a head of
+// kSchedHead words -- [0] the slot holding the last render's record, [1] the
+// geometry tag of that render (bit 31 set, which no other word of the buffer
+// reaches), [2] XCDs finished in the current render, [4 + 16x + k] the
+// record's class counts, and from word 256 one counter of finished blocks
+// per XCD, each in its own 128-byte line -- then three slots, each of 128
+// class counters (XCD x, class k: one line each) and 128 lists of C block
+// ids.  A render whose tag matches [1] launches XCD x's slots in the order
+// of the record in slot r = [0] when x's counts sum to its block count (else
+// in the fixed order: the same blocks either way), records into slot r+1
+// (cleared by the render before) and clears slot r+2; any other tag (a
+// zero-filled buffer, one last used at another size) means the fixed order,
+// and the render clears all three slots.  A block reads the head in its
+// first wave before a barrier; the first wave that finishes last copies the
+// record's counts to the head (atomic reads: every block took its count
+// before its XCD counter), writes [0] and [1] and resets the counters.
+// Counters are spread over lines and counted per XCD because atomics on one
+// address serialise: a counter taken by every wave of a 1080p launch (32,400)
+// made the frame 480 us instead of 285.
+constexpr uint32_t kSchedClasses = 16u;
+constexpr uint32_t kSchedXcds = 8u;
+constexpr uint32_t kSchedLists = kSchedXcds * kSchedClasses;  // 128
+constexpr uint32_t kSchedLine = 32u;                          // words per 128-byte line
+constexpr uint32_t kSchedCopy = 4u;                           // the record's counts in the head
+constexpr uint32_t kSchedXcdDone = 256u;                      // per-XCD block counters
+constexpr uint32_t kSchedHead = 512u;
+constexpr uint32_t kSchedCounts = kSchedLine * kSchedLists;   // counters of a slot
+static_assert(kSchedCopy + kSchedLists <= kSchedXcdDone &&
+                  kSchedXcdDone + kSchedLine * kSchedXcds <= kSchedHead &&
+                  kSchedCounts == 4096u,
+              "VT_SCHEDULE_WORDS layout");
+
+// The layout and tag of a launch's schedule, from the image size and tile
+// shape (kernel arguments the render kernels have anyway) and the grid.
+struct Sched {
+  uint32_t* p;       // vt_frame.schedule, or null: the fixed order
+  uint32_t C;        // list capacity: ceil(VT_SCHEDULE_BLOCKS(W, H) / 8)
+  uint32_t tag;      // geometry tag of this launch (bit 31 set)
+  uint32_t nblocks;  // blocks of this launch
+};
+__host__ __device__ __forceinline__ Sched sched_geometry(uint32_t* p, uint32_t W, uint32_t H,
+                                                         uint32_t log2_tw, uint32_t nblocks) {
+  Sched s;
+  s.p = p;
+  s.C = (VT_SCHEDULE_BLOCKS(W, H) + kSchedXcds - 1u) / kSchedXcds;
+  s.tag = 0x80000000u | ((W * 0x9E3779B1u) ^ (H * 0x85EBCA77u) ^ (log2_tw * 0xC2B2AE3Du));
+  s.nblocks = nblocks;
+  return s;
+}
+
+// Launch slots on XCD x (blockIdx.x % 8 == x): the blocks the fixed order
+// gives that XCD.
+__host__ __device__ __forceinline__ uint32_t sched_xcd_blocks(uint32_t nblocks, uint32_t x) {
+  return x < nblocks ? (nblocks - x + kSchedXcds - 1u) / kSchedXcds : 0u;
+}
+
+// How a block takes part: kSchedClear (the buffer is restarted) or
+// kSchedRecord (its cost goes to slot rslot + 1).
+enum : uint32_t { kSchedClear = 1, kSchedRecord = 2 };
+
+__device__ __forceinline__ uint32_t* sched_slot(const Sched& s, uint32_t k) {
+  return s.p + kSchedHead + k * (kSchedCounts + kSchedLists * s.C);
+}
+// The counter and the list of XCD x, class k in a slot.
+__device__ __forceinline__ uint32_t* sched_count(uint32_t* slot, uint32_t x, uint32_t k) {
+  return slot + (x * kSchedClasses + k) * kSchedLine;
+}
+__device__ __forceinline__ uint32_t* sched_list(uint32_t* slot, const Sched& s, uint32_t x,
+                                                uint32_t k) {
+  return slot + kSchedCounts + (x * kSchedClasses + k) * s.C;
+}
+
+// Block state kept in LDS from the block's start to the end of its first
+// wave, so that the traversal in between holds no extra registers: the
+// launch's schedule, the image block, the mode and read slot (mode | slot <<
+// 2) and the start time (low 32 bits of the 100 MHz clock).
+struct SchedBlock {
+  Sched s;
+  uint32_t bid, mode_slot, t0, last;
+};
+
+// The image block this launch slot renders: rank blockIdx.x / 8 of its XCD's
+// recorded order, or `fixed`.  The order is class by class from the
+// costliest and, within a class, the order in which the blocks finished
+// (1080p: 245 us per frame against 249 for the reverse).  The words read do
+// not change during the launch, so every block of an XCD decides alike.
+__device__ __forceinline__ uint32_t sched_block(const Sched& s, uint32_t fixed, uint32_t& mode,
+                                                uint32_t& rslot) {
+  rslot = 0u;
+  const uint32_t rot = s.p[0], tag = s.p[1];
+  if (tag != s.tag) {
+    mode = kSchedClear;
+    return fixed;
+  }
+  mode = kSchedRecord;
+  rslot = rot % 3u;
+  const uint32_t x = blockIdx.x % kSchedXcds;
+  const uint32_t* cnt = s.p + kSchedCopy + x * kSchedClasses;
+  uint32_t sum = 0u;
+#pragma unroll
+  for (uint32_t k = 0; k < kSchedClasses; ++k) sum += cnt[k];
+  if (sum != sched_xcd_blocks(s.nblocks, x)) return fixed;
+  uint32_t* R = sched_slot(s, rslot);
+  uint32_t rank = blockIdx.x / kSchedXcds;
+  for (int k = (int)kSchedClasses - 1; k >= 0; --k) {
+    const uint32_t c = cnt[k];
+    if (rank < c) {
+      const uint32_t b = sched_list(R, s, x, (uint32_t)k)[rank];
+      return b < s.nblocks ? b : fixed;
+    }
+    rank -= c;
+  }
+  return fixed;
+}
+
+// Start of a render kernel with a schedule (every thread): the image block
+// this launch slot renders; the first wave decides and passes the block to
+// the others through LDS.
+__device__ __forceinline__ uint32_t sched_begin(const Sched& s, uint32_t fixed, uint32_t wave,
+                                                SchedBlock& sb) {
+  if (wave == 0u) {
+    uint32_t mode, rslot;
+    const uint32_t bid = sched_block(s, fixed, mode, rslot);
+    sb.s = s;
+    sb.bid = bid;
+    sb.mode_slot = mode | (rslot << 2);
+  }
+  __syncthreads();
+  const uint32_t bid = __builtin_amdgcn_readfirstlane(sb.bid);  // uniform: a scalar
+  if (wave == 0u) sb.t0 = (uint32_t)wall_clock64();
+  return bid;
+}
+
+// End of a render kernel with a schedule (every thread).  In the first wave, lane 0: in
+// record mode the block goes on its XCD's list of its log2 cost class and
+// the launch's first blocks clear the next render's counters (in clear mode
+// those of all three slots); then the block counts itself on its XCD's
+// counter, and the block completing the last XCD has its wave publish the
+// record.
+__device__ __forceinline__ void sched_end(SchedBlock& sb, uint32_t wave, uint32_t lane) {
+  if (wave != 0u) return;
+  const Sched s = sb.s;
+  const uint32_t mode = sb.mode_slot & 3u, rslot = sb.mode_slot >> 2, bid = sb.bid;
+  const uint32_t x = blockIdx.x % kSchedXcds;
+  uint32_t* W = sched_slot(s, (rslot + 1u) % 3u);
+  if (lane == 0u) {
+    uint32_t last = 0u;
+    if (mode == kSchedRecord) {
+      const uint32_t d = (uint32_t)wall_clock64() - sb.t0;
+      const uint32_t lg = 31u - (uint32_t)__clz(d | 1u);
+      const uint32_t cls = lg < 5u ? 0u : min(lg - 5u, kSchedClasses - 1u);
+      const uint32_t pos = atomicAdd(sched_count(W, x, cls), 1u);
+      if (pos < s.C) sched_list(W, s, x, cls)[pos] = bid;
+      uint32_t* Z = sched_slot(s, (rslot + 2u) % 3u);
+      for (uint32_t q = blockIdx.x; q < kSchedLists; q += s.nblocks) Z[q * kSchedLine] = 0u;
+    } else {
+      for (uint32_t q = blockIdx.x; q < 3u * kSchedLists; q += s.nblocks)
+        sched_slot(s, q / kSchedLists)[(q % kSchedLists) * kSchedLine] = 0u;
+    }
+    // the XCD counter is taken after the count returned (pos is used above)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    const uint32_t xcds = min(s.nblocks, kSchedXcds);
+    if (atomicAdd(&s.p[kSchedXcdDone + x * kSchedLine], 1u) ==
+            sched_xcd_blocks(s.nblocks, x) - 1u &&
+        atomicAdd(&s.p[2], 1u) == xcds - 1u)
+      last = 1u;
+    sb.last = last;
+  }
+  __builtin_amdgcn_wave_barrier();
+  if (sb.last) {  // the whole first wave of the last block
+    for (uint32_t q = lane; q < kSchedLists; q += 64u)
+      s.p[kSchedCopy + q] =
+          mode == kSchedRecord
+              ? __hip_atomic_load(&W[q * kSchedLine], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+              : 0u;
+    if (lane < kSchedXcds) s.p[kSchedXcdDone + lane * kSchedLine] = 0u;
+    if (lane == 0u) {
+      s.p[0] = mode == kSchedRecord ? (rslot + 1u) % 3u : 2u;
+      s.p[1] = s.tag;
+      s.p[2] = 0u;
+    }
+  }
+}
+
 struct Primary {
   bool hit;
   int face;

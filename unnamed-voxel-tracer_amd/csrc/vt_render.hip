@@ -34,14 +34,20 @@ namespace vt {
 #ifndef VT_REF_WAVES_PER_SIMD
 #define VT_REF_WAVES_PER_SIMD 7
 #endif
-template <class L, bool kStats, int kShade>
+// kSched: the launch order comes from the frame's tile schedule (`sched`);
+// the kernels without it carry none of its code.
+template <class L, bool kStats, int kShade, bool kSched>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(kShade == kShadeRef ? VT_REF_WAVES_PER_SIMD : 7))) void k_render_dense(
-    DenseWorld w, ViewConsts v, ShadeConsts sc, FrameOut out, uint32_t log2_tw,
+    DenseWorld w, ViewConsts v, ShadeConsts sc, FrameOut out, uint32_t* sched, uint32_t log2_tw,
     uint32_t blocks_x) {
-  const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
+  __shared__ SchedBlock sb;
+  const uint32_t wave = threadIdx.x >> 6;
+  uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
+  if constexpr (kSched)
+    bid = sched_begin(sched_geometry(sched, v.W, v.H, log2_tw, gridDim.x), bid, wave, sb);
   const uint32_t tw = 1u << log2_tw, th = 64u >> log2_tw;
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63u;
   const uint32_t bx = bid % blocks_x, by = bid / blocks_x;
   const uint32_t px = bx * (2u * tw) + (wave & 1u) * tw + (lane & (tw - 1u));
   const uint32_t py = by * (2u * th) + (wave >> 1) * th + (lane >> log2_tw);
@@ -54,6 +60,7 @@ __attribute__((amdgpu_waves_per_eu(kShade == kShadeRef ? VT_REF_WAVES_PER_SIMD :
     flush_albedo(sc, gacc);
     flush_grad(sc, gacc);
   }
+  if constexpr (kSched) sched_end(sb, wave, lane);
 }
 
 // Seven waves per SIMD (at most 72 VGPRs): the engine's own frame 0.373 vs
@@ -61,13 +68,18 @@ __attribute__((amdgpu_waves_per_eu(kShade == kShadeRef ? VT_REF_WAVES_PER_SIMD :
 // the C1 world in this layout 0.823 vs 0.820 ms; five waves (90 VGPRs) were
 // slower on both.
 // kFree: the empty-space loops test the free-region map.
-template <bool kStats, bool kFree = false>
+// kSched: as k_render_dense.
+template <bool kStats, bool kFree, bool kSched>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_render_brickmap(
-    BrickmapWorld w, ViewConsts v, ShadeConsts sc, FrameOut out, uint32_t log2_tw,
-    uint32_t blocks_x) {
-  const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
+    BrickmapWorld w, ViewConsts v, ShadeConsts sc, FrameOut out, uint32_t* sched,
+    uint32_t log2_tw, uint32_t blocks_x) {
+  __shared__ SchedBlock sb;
+  const uint32_t wave = threadIdx.x >> 6;
+  uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
+  if constexpr (kSched)
+    bid = sched_begin(sched_geometry(sched, v.W, v.H, log2_tw, gridDim.x), bid, wave, sb);
   const uint32_t tw = 1u << log2_tw, th = 64u >> log2_tw;
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63u;
   const uint32_t bx = bid % blocks_x, by = bid / blocks_x;
   const uint32_t px = bx * (2u * tw) + (wave & 1u) * tw + (lane & (tw - 1u));
   const uint32_t py = by * (2u * th) + (wave >> 1) * th + (lane >> log2_tw);
@@ -138,6 +150,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     }
   }
   if (kStats) flush_counters(out, cnt);
+  if constexpr (kSched) sched_end(sb, wave, lane);
 }
 
 static void grid_shape(const ViewConsts& v, TileShape tile, uint32_t& blocks_x, uint32_t& nblocks) {
@@ -147,28 +160,69 @@ static void grid_shape(const ViewConsts& v, TileShape tile, uint32_t& blocks_x, 
   nblocks = blocks_x * blocks_y;
 }
 
-template <class L, bool kStats>
+
+template <class L, bool kStats, bool kSched>
 static void launch_dense_shade(const DenseWorld& w, const ViewConsts& v, const ShadeConsts& sc,
-                               const FrameOut& out, uint32_t log2_w, uint32_t blocks_x, dim3 grid,
-                               hipStream_t stream) {
+                               const FrameOut& out, uint32_t* sd, uint32_t log2_w,
+                               uint32_t blocks_x, dim3 grid, hipStream_t stream) {
   switch (shade_mode(sc.flags)) {
     case kShadeLambertGrad:
-      hipLaunchKernelGGL((k_render_dense<L, kStats, kShadeLambertGrad>), grid, dim3(256), 0,
-                         stream, w, v, sc, out, log2_w, blocks_x);
+      hipLaunchKernelGGL((k_render_dense<L, kStats, kShadeLambertGrad, kSched>), grid, dim3(256),
+                         0, stream, w, v, sc, out, sd, log2_w, blocks_x);
       break;
     case kShadeLambert:
-      hipLaunchKernelGGL((k_render_dense<L, kStats, kShadeLambert>), grid, dim3(256), 0, stream, w,
-                         v, sc, out, log2_w, blocks_x);
+      hipLaunchKernelGGL((k_render_dense<L, kStats, kShadeLambert, kSched>), grid, dim3(256), 0,
+                         stream, w, v, sc, out, sd, log2_w, blocks_x);
       break;
     default:
-      hipLaunchKernelGGL((k_render_dense<L, kStats, kShadeRef>), grid, dim3(256), 0, stream, w, v,
-                         sc, out, log2_w, blocks_x);
+      hipLaunchKernelGGL((k_render_dense<L, kStats, kShadeRef, kSched>), grid, dim3(256), 0,
+                         stream, w, v, sc, out, sd, log2_w, blocks_x);
   }
 }
 
+// The traversal variant of a dense launch (kSched: as k_render_dense).
+template <bool kSched>
+static void launch_dense_select(const DenseWorld& w, uint32_t traversal, const ViewConsts& v,
+                                const ShadeConsts& sc, const FrameOut& out, uint32_t* sd,
+                                uint32_t lw, uint32_t blocks_x, dim3 grid, hipStream_t stream) {
+  const bool stats = out.stats != nullptr;
+  if (traversal == VT_TRAVERSAL_BITMASK) {
+    if (stats)
+      launch_dense_shade<BitmaskLookup, true, kSched>(w, v, sc, out, sd, lw, blocks_x, grid, stream);
+    else
+      launch_dense_shade<BitmaskLookup, false, kSched>(w, v, sc, out, sd, lw, blocks_x, grid, stream);
+  } else if (traversal == VT_TRAVERSAL_BITMASK8) {
+    if (stats)
+      launch_dense_shade<BitmaskLookup8, true, kSched>(w, v, sc, out, sd, lw, blocks_x, grid, stream);
+    else
+      launch_dense_shade<BitmaskLookup8, false, kSched>(w, v, sc, out, sd, lw, blocks_x, grid, stream);
+  } else {
+    const bool long_caps = sc.max_primary >= kLongCap || sc.max_shadow >= kLongCap;
+    if (stats && w.free_bytes)  // statistics of the steps executed
+      launch_dense_shade<FlatLookupFree, true, kSched>(w, v, sc, out, sd, lw, blocks_x, grid, stream);
+    else if (stats)
+      launch_dense_shade<FlatLookup, true, kSched>(w, v, sc, out, sd, lw, blocks_x, grid, stream);
+    else if (w.free_bytes)
+      launch_dense_shade<FlatLookupFree, false, kSched>(w, v, sc, out, sd, lw, blocks_x, grid, stream);
+    else if (long_caps)
+      launch_dense_shade<FlatLookupLong, false, kSched>(w, v, sc, out, sd, lw, blocks_x, grid, stream);
+    else
+      launch_dense_shade<FlatLookup, false, kSched>(w, v, sc, out, sd, lw, blocks_x, grid, stream);
+  }
+}
+
+// The tile schedule is taken for frames with a buffer and a step cap of at
+// least kLongCap.  With the shaders' own caps (192 / 48) the blocks' costs
+// are flat and the costliest-first order measured slower: the engine's frame
+// +8.6% at 1280x720, +19% at 1920x1080, the 512^3 world +4% at 1280x720;
+// with caps of 2048 it gains 7-25% on both worlds at 1280x720..2560x1440.
+static bool sched_taken(const uint32_t* schedule, const ShadeConsts& sc) {
+  return schedule && (sc.max_primary >= kLongCap || sc.max_shadow >= kLongCap);
+}
+
 hipError_t launch_render_dense(const DenseWorld& w_in, uint32_t traversal, const ViewConsts& v,
-                               const ShadeConsts& sc_in, const FrameOut& out, TileShape tile,
-                               uint32_t compute_units, hipStream_t stream) {
+                               const ShadeConsts& sc_in, const FrameOut& out, uint32_t* schedule,
+                               TileShape tile, uint32_t compute_units, hipStream_t stream) {
   (void)compute_units;
   DenseWorld w = w_in;
   if (out.steps) w.free_bytes = 0;  // step counts are stored: no early end of rays
@@ -176,32 +230,32 @@ hipError_t launch_render_dense(const DenseWorld& w_in, uint32_t traversal, const
   sc.shadow = make_ray_consts(sc.sun[0], sc.sun[1], sc.sun[2], w.mx, w.my, w.mz);
   uint32_t blocks_x, nblocks;
   grid_shape(v, tile, blocks_x, nblocks);
+  if (schedule && nblocks > VT_SCHEDULE_BLOCKS(v.W, v.H)) return hipErrorInvalidValue;
   const dim3 grid(nblocks);
-  const bool stats = out.stats != nullptr;
-  const uint32_t lw = tile.log2_w;
-  if (traversal == VT_TRAVERSAL_BITMASK) {
-    if (stats) launch_dense_shade<BitmaskLookup, true>(w, v, sc, out, lw, blocks_x, grid, stream);
-    else launch_dense_shade<BitmaskLookup, false>(w, v, sc, out, lw, blocks_x, grid, stream);
-  } else if (traversal == VT_TRAVERSAL_BITMASK8) {
-    if (stats) launch_dense_shade<BitmaskLookup8, true>(w, v, sc, out, lw, blocks_x, grid, stream);
-    else launch_dense_shade<BitmaskLookup8, false>(w, v, sc, out, lw, blocks_x, grid, stream);
-  } else {
-    const bool long_caps = sc.max_primary >= kLongCap || sc.max_shadow >= kLongCap;
-    if (stats && w.free_bytes)  // statistics of the steps executed
-      launch_dense_shade<FlatLookupFree, true>(w, v, sc, out, lw, blocks_x, grid, stream);
-    else if (stats) launch_dense_shade<FlatLookup, true>(w, v, sc, out, lw, blocks_x, grid, stream);
-    else if (w.free_bytes)
-      launch_dense_shade<FlatLookupFree, false>(w, v, sc, out, lw, blocks_x, grid, stream);
-    else if (long_caps)
-      launch_dense_shade<FlatLookupLong, false>(w, v, sc, out, lw, blocks_x, grid, stream);
-    else launch_dense_shade<FlatLookup, false>(w, v, sc, out, lw, blocks_x, grid, stream);
-  }
+  if (sched_taken(schedule, sc))
+    launch_dense_select<true>(w, traversal, v, sc, out, schedule, tile.log2_w, blocks_x, grid,
+                              stream);
+  else
+    launch_dense_select<false>(w, traversal, v, sc, out, nullptr, tile.log2_w, blocks_x, grid,
+                               stream);
   return hipGetLastError();
 }
 
+template <bool kStats, bool kFree>
+static void launch_brickmap(bool sched, const BrickmapWorld& w, const ViewConsts& v,
+                            const ShadeConsts& sc, const FrameOut& out, uint32_t* schedule,
+                            uint32_t lw, uint32_t blocks_x, dim3 grid, hipStream_t stream) {
+  if (sched)
+    hipLaunchKernelGGL((k_render_brickmap<kStats, kFree, true>), grid, dim3(256), 0, stream, w, v,
+                       sc, out, schedule, lw, blocks_x);
+  else
+    hipLaunchKernelGGL((k_render_brickmap<kStats, kFree, false>), grid, dim3(256), 0, stream, w,
+                       v, sc, out, nullptr, lw, blocks_x);
+}
+
 hipError_t launch_render_brickmap(const BrickmapWorld& w_in, const ViewConsts& v,
-                                  const ShadeConsts& sc, const FrameOut& out, TileShape tile,
-                                  uint32_t compute_units, hipStream_t stream) {
+                                  const ShadeConsts& sc, const FrameOut& out, uint32_t* schedule,
+                                  TileShape tile, uint32_t compute_units, hipStream_t stream) {
   (void)compute_units;
   BrickmapWorld w = w_in;
   // step counts stored, or statistics (of traceMap's steps on this layout):
@@ -209,16 +263,16 @@ hipError_t launch_render_brickmap(const BrickmapWorld& w_in, const ViewConsts& v
   if (out.steps || out.stats) w.free_bytes = 0;
   uint32_t blocks_x, nblocks;
   grid_shape(v, tile, blocks_x, nblocks);
-  dim3 grid(nblocks), block(256);
+  if (schedule && nblocks > VT_SCHEDULE_BLOCKS(v.W, v.H)) return hipErrorInvalidValue;
+  const dim3 grid(nblocks);
+  const bool sched = sched_taken(schedule, sc);
+  const uint32_t lw = tile.log2_w;
   if (out.stats)
-    hipLaunchKernelGGL((k_render_brickmap<true>), grid, block, 0, stream, w, v, sc, out,
-                       tile.log2_w, blocks_x);
+    launch_brickmap<true, false>(sched, w, v, sc, out, schedule, lw, blocks_x, grid, stream);
   else if (w.occ && w.free_bytes)
-    hipLaunchKernelGGL((k_render_brickmap<false, true>), grid, block, 0, stream, w, v, sc, out,
-                       tile.log2_w, blocks_x);
+    launch_brickmap<false, true>(sched, w, v, sc, out, schedule, lw, blocks_x, grid, stream);
   else
-    hipLaunchKernelGGL((k_render_brickmap<false>), grid, block, 0, stream, w, v, sc, out,
-                       tile.log2_w, blocks_x);
+    launch_brickmap<false, false>(sched, w, v, sc, out, schedule, lw, blocks_x, grid, stream);
   return hipGetLastError();
 }
 

@@ -69,7 +69,7 @@ class Frame(C.Structure):
     _fields_ = [("shaded", C.c_void_p), ("hits", C.c_void_p), ("color", C.c_void_p),
                 ("normal", C.c_void_p), ("position", C.c_void_p),
                 ("illumination", C.c_void_p), ("steps", C.c_void_p), ("stats", C.c_void_p),
-                ("target", C.c_void_p), ("display", C.c_void_p)]
+                ("target", C.c_void_p), ("display", C.c_void_p), ("schedule", C.c_void_p)]
 
 
 class View(C.Structure):
@@ -91,10 +91,21 @@ class PickResult(C.Structure):
 VT_TILE_8X8, VT_TILE_16X4, VT_TILE_32X2 = 3, 4, 5
 VT_NOALLOC_MAX_VIEWS = 256
 
+
+def schedule_blocks(width: int, height: int) -> int:
+    """VT_SCHEDULE_BLOCKS(W, H) of include/vt.h."""
+    return ((width + 15) // 16) * ((height + 7) // 8)
+
+
+def schedule_words(width: int, height: int) -> int:
+    """VT_SCHEDULE_WORDS(W, H) of include/vt.h: words of a frame's tile
+    schedule buffer."""
+    return 512 + 3 * (4096 + 128 * ((schedule_blocks(width, height) + 7) // 8))
+
 assert C.sizeof(Camera) == 96
 assert C.sizeof(RenderParams) == 72
-assert C.sizeof(Frame) == 80
-assert C.sizeof(View) == 192
+assert C.sizeof(Frame) == 88
+assert C.sizeof(View) == 200
 assert C.sizeof(PickResult) == 64
 
 

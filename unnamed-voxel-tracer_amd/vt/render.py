@@ -248,10 +248,15 @@ class Frame:
     stats: object = None
     target: object = None
     display: object = None
+    schedule: object = None
 
     @classmethod
     def alloc(cls, width, height, device=0, shaded=True, hits=True, gbuffer=False, steps=False,
-              stats=False, display=False):
+              stats=False, display=False, schedule=True):
+        """Device buffers for a width x height view.  `schedule`: a zeroed
+        tile schedule buffer (vt_frame.schedule), so that each render into
+        this frame launches its blocks in the cost order the previous render
+        measured."""
         torch = _torch()
         dev = f"cuda:{device}"
         f = cls(width, height)
@@ -268,6 +273,9 @@ class Frame:
             f.stats = torch.zeros(8, dtype=torch.int64, device=dev)
         if display:
             f.display = torch.empty((height, width), dtype=torch.int32, device=dev)
+        if schedule:
+            f.schedule = torch.zeros(N.schedule_words(width, height), dtype=torch.int32,
+                                     device=dev)
         return f
 
     def check(self, device, who="frame"):
@@ -279,12 +287,14 @@ class Frame:
         check_tensor(self.steps, f"{who}.steps", "int32", n, device)
         check_tensor(self.display, f"{who}.display", "int32", n, device)
         check_tensor(self.stats, f"{who}.stats", "int64", 8, device)
+        check_tensor(self.schedule, f"{who}.schedule", "int32",
+                     N.schedule_words(self.width, self.height), device)
 
     def struct(self) -> N.Frame:
         p = lambda t: None if t is None else t.data_ptr()
         return N.Frame(p(self.shaded), p(self.hits), p(self.color), p(self.normal),
                        p(self.position), p(self.illumination), p(self.steps), p(self.stats),
-                       p(self.target), p(self.display))
+                       p(self.target), p(self.display), p(self.schedule))
 
     def stats_dict(self):
         s = self.stats.cpu().numpy()
