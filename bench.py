@@ -167,6 +167,8 @@ def bench_batched(args):
         ex = st  # the bitmask traversals run every ray to its end
     abytes = ex["primary_steps"] + ex["shadow_steps"] + OUT_BYTES_PER_PIXEL * npix
     achieved = abytes / (ms * 1e-3) / 1e9
+    bkey = ("k_render_batch/flat_free/1024cube_64views"
+            if free and TRAVERSALS[best.traversal] == "flat" else None)
     line = {
         "metric": "Mrays/s primary+shadow, 64 uneven cameras on a 1024³ grid, one persistent "
                   "graph launch (vs one plain launch per view)",
@@ -199,7 +201,10 @@ def bench_batched(args):
             "speedup_vs_plain": round(plain_ms / ms, 3),
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     # counters of the flat free-region batch kernel (the tuning picked here)
+                     "traffic": measured_traffic(bkey) if bkey else None,
+                     "issue": measured_issue(bkey, ms) if bkey else None},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -251,6 +256,7 @@ def bench_brickmap(args):
     # per block: the literal loop ends at the hit block), whose statistics
     # count them
     free = vt.free_map_used(world, f, prm)
+    mkey = "k_render_brickmap/free/512cube_1080p"
     ex = st
     if free:
         ex = counted_stats(vt, torch, ctx, dense,
@@ -288,7 +294,9 @@ def bench_brickmap(args):
             "executed_steps": ex["primary_steps"] + ex["shadow_steps"],
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": measured_traffic(mkey) if free else None,
+                     "issue": measured_issue(mkey, ms) if free else None,
                      "bytes": "lower bound: 1 B per executed step + 28 B per pixel"},
         "cpu_baseline": None,
     }
@@ -744,7 +752,9 @@ def bench_engine(args):
             "hits_equal_oracle": same,
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": measured_traffic("k_render_brickmap/engine/1280x720"),
+                     "issue": measured_issue("k_render_brickmap/engine/1280x720", ms),
                      "bytes": "lower bound: 1 B per step + 28 B per pixel",
                      "literal_loop_gbs": round(literal_bytes / (ms * 1e-3) / 1e9, 2)},
         "cpu_baseline": cpu,
