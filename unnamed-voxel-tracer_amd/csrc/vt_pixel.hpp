@@ -193,6 +193,10 @@ __device__ __forceinline__ void sched_end(SchedBlock& sb, uint32_t wave, uint32_
   if (lane == 0u) {
     uint32_t last = 0u;
     if (mode == kSchedRecord) {
+      // octaves of 100 MHz ticks from 2^5 (0.32 us) to 2^20 (10.5 ms).  Measured at
+      // 1080p: octaves 243 us, half octaves 243, half octaves of the first
+      // wave's most executed steps instead of its time 248 (better at
+      // 1280x720: 138 against 148 us).
       const uint32_t d = (uint32_t)wall_clock64() - sb.t0;
       const uint32_t lg = 31u - (uint32_t)__clz(d | 1u);
       const uint32_t cls = lg < 5u ? 0u : min(lg - 5u, kSchedClasses - 1u);
