@@ -363,3 +363,85 @@ def test_flag_rules(ctx):
         vt.render(ctx, w, cam, f, vt.params(8, 8, 64, 64, S.SUN, ON | OFF))
     assert e.value.status == vt._native.VT_ERR_INVALID_ARGUMENT
     w.close()
+
+
+def grazing_cameras(n, y_top, rng, count):
+    """Eyes just above (or below) a slab's face, looking across it at small
+    pitches in every compass direction: rays that pass the box of the nonempty
+    voxels by fractions of a voxel, hit it far away, or leave the grid."""
+    out = []
+    for k in range(count):
+        eye = np.array([rng.uniform(0.0, n), y_top + rng.uniform(0.2, 3.0), rng.uniform(0.0, n)])
+        if k % 3 == 2:  # from outside the grid
+            eye[[0, 2]] = rng.choice([-0.3 * n, 1.3 * n], 2)
+        # toward the middle of the grid, give or take 60 degrees
+        yaw = np.arctan2(0.5 * n - eye[2], 0.5 * n - eye[0]) + rng.uniform(-1.05, 1.05)
+        pitch = rng.uniform(-0.05, 0.05)
+        tgt = eye + np.array([np.cos(yaw), np.tan(pitch), np.sin(yaw)]) * n
+        out.append(vcam.look_at(tuple(eye), tuple(tgt), float(rng.uniform(20, 60))))
+    return out
+
+
+@pytest.mark.parametrize("n", [64, 256])
+def test_reach_box_grazing_rays(ctx, n):
+    """Primary rays whose line misses the box of the nonempty voxels
+    (widened by the traversal's reach) end as misses before the walk: a slab
+    seen from just above its top face at grazing angles, so that many lines
+    pass the box by less than the widening and others by more.  Every output
+    but the step counts equals the oracle's."""
+    pal = S.palette()
+    rng = np.random.default_rng(77 + n)
+    g = np.zeros((n, n, n), np.uint8)
+    y0, y1 = n // 4, n // 4 + 3
+    g[n // 8:7 * n // 8, y0:y1, n // 8:7 * n // 8] = 4
+    g[n // 2, y1:y1 + n // 8, n // 2] = 6  # a post on it
+    wd = vt.World.dense(ctx, g, pal)
+    ow = O.DenseWorld(g, pal)
+    W, H = 64, 48
+    cap = 4 * n
+    hit_share = []
+    for k, cam in enumerate(grazing_cameras(n, float(y1), rng, 9)):
+        sun = SUNS[k % len(SUNS)]
+        o = O.render(ow, cam, W, H, cap, cap, sun)
+        label = f"grazing n={n} camera {k}"
+        log(label)
+        compare(render_free(ctx, wd, cam, W, H, cap, cap, sun), o, stats=False, label=label)
+        hit_share.append(float(np.mean(o["hits"][..., 0] != 0xFFFFFFFF)))
+    log(f"grazing n={n}: hit shares {[round(x, 2) for x in hit_share]}")
+    assert 0.05 < np.mean(hit_share) < 0.95
+    wd.close()
+
+
+def test_reach_box_grows_with_edits(ctx):
+    """The box follows edits: voxels set far outside it after the world was
+    made are hit (rays toward them are not ended by the stale box), and a
+    world made empty and filled again by edits renders like the oracle."""
+    pal = S.palette()
+    n = 128
+    g = np.zeros((n, n, n), np.uint8)
+    g[10:20, 2:5, 10:20] = 3
+    w = vt.World.dense(ctx, g, pal)
+    cam = vcam.look_at((64.0, 70.0, -40.0), (64.0, 90.0, 64.0), 50.0)
+    W, H = 64, 48
+    o = O.render(O.DenseWorld(g, pal), cam, W, H, 512, 512, S.SUN)
+    compare(render_free(ctx, w, cam, W, H, 512, 512, S.SUN), o, stats=False, label="small box")
+    # a wall high above the box, in front of the camera
+    z, y, x = np.mgrid[80:82, 80:110, 40:90]
+    xyz = np.stack([x.ravel(), y.ravel(), z.ravel()], 1).astype(np.uint32)
+    w.set_voxels(xyz, np.full(len(xyz), 7, np.uint32))
+    g[80:82, 80:110, 40:90] = 7
+    o = O.render(O.DenseWorld(g, pal), cam, W, H, 512, 512, S.SUN)
+    assert np.mean(o["hits"][..., 0] != 0xFFFFFFFF) > 0.1
+    compare(render_free(ctx, w, cam, W, H, 512, 512, S.SUN), o, stats=False, label="wall added")
+    w.close()
+    # an empty world (no box: nothing ends early) filled by edits
+    g = np.zeros((n, n, n), np.uint8)
+    w = vt.World.dense(ctx, g, pal)
+    o = O.render(O.DenseWorld(g, pal), cam, W, H, 512, 512, S.SUN)
+    compare(render_free(ctx, w, cam, W, H, 512, 512, S.SUN), o, stats=False, label="empty")
+    w.set_voxels(xyz, np.full(len(xyz), 7, np.uint32))
+    g[80:82, 80:110, 40:90] = 7
+    o = O.render(O.DenseWorld(g, pal), cam, W, H, 512, 512, S.SUN)
+    compare(render_free(ctx, w, cam, W, H, 512, 512, S.SUN), o, stats=False,
+            label="empty world, wall added")
+    w.close()

@@ -3,6 +3,7 @@
 // Validation happens here, on the host, before anything is enqueued: every
 // size, pointer and alignment a kernel relies on is checked so that no launch
 // can index outside the buffers it is given.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -118,7 +119,43 @@ DenseWorld dense_view(const vt_world* w) {
   d.freemap = w->freemap;
   d.free_oct_bytes = free_words_per_octant(w->dim) * 4u;
   d.free_bytes = w->freemap && w->free_valid ? 8u * d.free_oct_bytes : 0u;
+  const float m = wide_margin(w->dim);
+  for (int i = 0; i < 3; i++) {
+    d.reach_lo[i] = w->box_any ? (float)w->box_lo[i] - m : 0.0f;
+    d.reach_hi[i] = w->box_any ? (float)w->box_hi[i] + 1.0f + m : (float)w->dim;
+  }
   return d;
+}
+
+// The box of the nonempty voxels of a linear grid (x + dim (y + dim z)).
+void scan_dense_box(const uint8_t* grid, uint32_t dim, vt_world* w) {
+  w->box_any = false;
+  for (int i = 0; i < 3; i++) {
+    w->box_lo[i] = (int32_t)dim;
+    w->box_hi[i] = -1;
+  }
+  for (uint32_t z = 0; z < dim; z++)
+    for (uint32_t y = 0; y < dim; y++) {
+      const uint8_t* row = grid + (size_t)dim * (y + (size_t)dim * z);
+      uint64_t any = 0;
+      for (uint32_t x = 0; x + 8 <= dim; x += 8) {
+        uint64_t q;
+        std::memcpy(&q, row + x, 8);
+        any |= q;
+      }
+      for (uint32_t x = dim & ~7u; x < dim; x++) any |= row[x];
+      if (!any) continue;
+      uint32_t x0 = 0, x1 = dim - 1;
+      while (!row[x0]) x0++;
+      while (!row[x1]) x1--;
+      w->box_any = true;
+      w->box_lo[0] = std::min(w->box_lo[0], (int32_t)x0);
+      w->box_hi[0] = std::max(w->box_hi[0], (int32_t)x1);
+      w->box_lo[1] = std::min(w->box_lo[1], (int32_t)y);
+      w->box_hi[1] = std::max(w->box_hi[1], (int32_t)y);
+      w->box_lo[2] = std::min(w->box_lo[2], (int32_t)z);
+      w->box_hi[2] = std::max(w->box_hi[2], (int32_t)z);
+    }
 }
 
 BrickmapWorld brickmap_view(const vt_world* w) {
@@ -387,6 +424,7 @@ int vt_world_create_dense_u8(vt_ctx* ctx, const uint8_t* grid, uint32_t dim,
     free_world(w);
     return fail(VT_ERR_OUT_OF_MEMORY, "vt_world_create_dense_u8: staging %zu bytes", n);
   }
+  vt::scan_dense_box(grid, dim, w);
   e = hipMemcpy(lin, grid, n, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(w->palette, palette_rgba8, 256 * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = vt::launch_build_dense(lin, dim, w->vox, w->l0, w->l1, w->l1b, nullptr);

@@ -348,6 +348,17 @@ int set_dense(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint32_t
   std::vector<uint64_t> addr;
   std::vector<uint32_t> val, bricks, regions, regions8;
   last.reserve(n);
+  // the box of the nonempty voxels grows with every voxel set nonzero (a
+  // cleared voxel leaves it as it was: still a box around all of them)
+  for (uint32_t i = 0; i < n; i++) {
+    if (values[i] == 0u) continue;
+    for (int a = 0; a < 3; a++) {
+      const int32_t c = (int32_t)xyz[3 * (size_t)i + a];
+      w->box_lo[a] = w->box_any ? std::min(w->box_lo[a], c) : c;
+      w->box_hi[a] = w->box_any ? std::max(w->box_hi[a], c) : c;
+    }
+    w->box_any = true;
+  }
   for (uint32_t i = 0; i < n; i++) {
     const uint32_t code =
         morton_code(xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2]);

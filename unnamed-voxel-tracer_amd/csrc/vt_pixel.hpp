@@ -117,14 +117,17 @@ __device__ __forceinline__ uint32_t* sched_list(uint32_t* slot, const Sched& s, 
   return slot + kSchedCounts + (x * kSchedClasses + k) * s.C;
 }
 
-// Block state kept in LDS from the block's start to the end of its first
+// Block state kept in LDS from the block's start to the end of its last
 // wave, so that the traversal in between holds no extra registers: the
 // launch's schedule, the image block, the mode and read slot (mode | slot <<
-// 2) and the start time (low 32 bits of the 100 MHz clock).
+// 2), the start time (low 32 bits of the 100 MHz clock) and the count of
+// the block's waves that have finished.
 struct SchedBlock {
   Sched s;
-  uint32_t bid, mode_slot, t0, last;
+  uint32_t bid, mode_slot, t0, last, done;
 };
+// Waves of a render block (256 threads).
+constexpr uint32_t kSchedWaves = 4;
 
 // The image block this launch slot renders: rank blockIdx.x / 8 of its XCD's
 // recorded order, or `fixed`.  The order is class by class from the
@@ -171,21 +174,28 @@ __device__ __forceinline__ uint32_t sched_begin(const Sched& s, uint32_t fixed, 
     sb.s = s;
     sb.bid = bid;
     sb.mode_slot = mode | (rslot << 2);
+    sb.done = 0u;
+    sb.t0 = (uint32_t)wall_clock64();
   }
   __syncthreads();
-  const uint32_t bid = __builtin_amdgcn_readfirstlane(sb.bid);  // uniform: a scalar
-  if (wave == 0u) sb.t0 = (uint32_t)wall_clock64();
-  return bid;
+  return __builtin_amdgcn_readfirstlane(sb.bid);  // uniform: a scalar
 }
 
-// End of a render kernel with a schedule (every thread).  In the first wave, lane 0: in
-// record mode the block goes on its XCD's list of its log2 cost class and
+// End of a render kernel with a schedule (every thread).  The block's last
+// wave to finish does the bookkeeping, so that the block's cost is the time
+// to its slowest wave (the first wave's own time misjudged blocks whose
+// first wave ends early while another runs long, and such blocks launched
+// last ran past the rest of the frame: 1080p 270 against 191 us).  Its lane
+// 0: in record mode the block goes on its XCD's list of its log2 cost class and
 // the launch's first blocks clear the next render's counters (in clear mode
 // those of all three slots); then the block counts itself on its XCD's
 // counter, and the block completing the last XCD has its wave publish the
 // record.
-__device__ __forceinline__ void sched_end(SchedBlock& sb, uint32_t wave, uint32_t lane) {
-  if (wave != 0u) return;
+__device__ __forceinline__ void sched_end(SchedBlock& sb, uint32_t lane) {
+  uint32_t prev = 0u;
+  if (lane == 0u)
+    prev = __hip_atomic_fetch_add(&sb.done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+  if (__builtin_amdgcn_readfirstlane(prev) != kSchedWaves - 1u) return;
   const Sched s = sb.s;
   const uint32_t mode = sb.mode_slot & 3u, rslot = sb.mode_slot >> 2, bid = sb.bid;
   const uint32_t x = blockIdx.x % kSchedXcds;
@@ -193,10 +203,11 @@ __device__ __forceinline__ void sched_end(SchedBlock& sb, uint32_t wave, uint32_
   if (lane == 0u) {
     uint32_t last = 0u;
     if (mode == kSchedRecord) {
-      // octaves of 100 MHz ticks from 2^5 (0.32 us) to 2^20 (10.5 ms).  Measured at
-      // 1080p: octaves 243 us, half octaves 243, half octaves of the first
-      // wave's most executed steps instead of its time 248 (better at
-      // 1280x720: 138 against 148 us).
+      // octaves of 100 MHz ticks from 2^5 (0.32 us) to 2^20 (10.5 ms), the
+      // block's time from its start to the end of its last wave.  Measured
+      // at 1080p with the first wave's time as the cost: octaves 243 us,
+      // half octaves 243, half octaves of the first wave's most executed
+      // steps instead of its time 248 (better at 1280x720: 138 against 148).
       const uint32_t d = (uint32_t)wall_clock64() - sb.t0;
       const uint32_t lg = 31u - (uint32_t)__clz(d | 1u);
       const uint32_t cls = lg < 5u ? 0u : min(lg - 5u, kSchedClasses - 1u);
@@ -218,7 +229,7 @@ __device__ __forceinline__ void sched_end(SchedBlock& sb, uint32_t wave, uint32_
     sb.last = last;
   }
   __builtin_amdgcn_wave_barrier();
-  if (sb.last) {  // the whole first wave of the last block
+  if (sb.last) {  // the whole last wave of the last block
     for (uint32_t q = lane; q < kSchedLists; q += 64u)
       s.p[kSchedCopy + q] =
           mode == kSchedRecord
@@ -641,9 +652,16 @@ __device__ __forceinline__ void trace_pixel_dense(const DenseWorld& w, const Vie
   float ex = (v.pos[0] + dx * tc) - 0.001f;
   float ey = (v.pos[1] + dy * tc) - 0.001f;
   float ez = (v.pos[2] + dz * tc) - 0.001f;
-  if (!dense_begin<kStats>(r, lk, w, ex, ey, ez, dx, dy, dz, sc.max_primary, h, cnt.l1f,
-                           cnt.l0f))
+  const RayConsts pk = make_ray_consts(dx, dy, dz, w.mx, w.my, w.mz);
+  if (L::kFree && !line_meets_reach(w, ex, ey, ez, pk)) {
+    // with the free-region test rays end early: this one cannot reach a
+    // nonempty voxel (a miss, no steps)
+    h.hit = false;
+    h.steps = 0;
+  } else if (!dense_begin_k<kStats>(r, lk, w, ex, ey, ez, pk, sc.max_primary, h, cnt.l1f,
+                                    cnt.l0f)) {
     dense_run<kStats>(r, lk, w, h, cnt.l1f, cnt.l0f);
+  }
   p.hit = h.hit;
   p.steps = h.steps;
   float sox = 0.f, soy = 0.f, soz = 0.f;

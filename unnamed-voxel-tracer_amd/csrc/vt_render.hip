@@ -60,7 +60,7 @@ __attribute__((amdgpu_waves_per_eu(kShade == kShadeRef ? VT_REF_WAVES_PER_SIMD :
     flush_albedo(sc, gacc);
     flush_grad(sc, gacc);
   }
-  if constexpr (kSched) sched_end(sb, wave, lane);
+  if constexpr (kSched) sched_end(sb, lane);
 }
 
 // Seven waves per SIMD (at most 72 VGPRs): the engine's own frame 0.373 vs
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     }
   }
   if (kStats) flush_counters(out, cnt);
-  if constexpr (kSched) sched_end(sb, wave, lane);
+  if constexpr (kSched) sched_end(sb, lane);
 }
 
 static void grid_shape(const ViewConsts& v, TileShape tile, uint32_t& blocks_x, uint32_t& nblocks) {

@@ -210,6 +210,9 @@ struct DenseWorld {
   // free-region map (see FreeMap below); free_bytes 0 = rays are not ended early
   const uint32_t* freemap;
   uint32_t free_bytes, free_oct_bytes;
+  // the box of the nonempty voxels widened by the traversal's reach
+  // (line_meets_reach); the whole grid when the world is empty
+  float reach_lo[3], reach_hi[3];
 };
 
 __device__ __forceinline__ uint32_t part1by2(uint32_t x) {
@@ -423,6 +426,35 @@ __device__ __forceinline__ void dense_ray_from(DenseRay& r, const RayConsts& k) 
 __device__ __forceinline__ void dense_ray_consts(DenseRay& r, const DenseWorld& w, float dx,
                                                  float dy, float dz) {
   dense_ray_from(r, make_ray_consts(dx, dy, dz, w.mx, w.my, w.mz));
+}
+
+// Can traceMap's block-level walk from e (voxel units) along d probe a
+// nonempty voxel?  Only if the line e + d t, t >= 0, meets the box of the
+// nonempty voxels widened by how far a probe can lie from that line:
+//   * every step moves the position along d up to binary32 rounding (< 1e-6
+//     sub-voxel per step: 0.01 voxel over 65,535 steps);
+//   * a step across a negative axis re-anchors within at 8 x 0.999 instead
+//     of 8, 0.001 voxel off the line per crossing, at most dim crossings per
+//     axis inside the grid (the walk ends where it leaves it);
+//   * the probed voxel is the position's, or the next one on a positive axis
+//     whose within reached 8: within 2 voxels of the position;
+// so the widening of 2.5 + dim / 1000 voxels (wide_margin) covers all three
+// with room for the rounding of this test.  A ray whose line misses it ends
+// as a miss; only its step count differs from traceMap's.
+__host__ __device__ __forceinline__ float wide_margin(uint32_t dim) {
+  return 2.5f + (float)dim / 1000.0f;
+}
+__device__ __forceinline__ bool line_meets_reach(const DenseWorld& w, float ex, float ey,
+                                                 float ez, const RayConsts& k) {
+  const float e[3] = {ex, ey, ez};
+  float tn = 0.0f, tf = 3.0e38f;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const float a = (w.reach_lo[i] - e[i]) * k.inv[i], b = (w.reach_hi[i] - e[i]) * k.inv[i];
+    tn = fmaxf(tn, fminf(a, b));
+    tf = fminf(tf, fmaxf(a, b));
+  }
+  return tn <= tf;
 }
 
 // Set-up (map.glsl:85-104) plus iteration 0 (stepSize 0).  Returns true when
