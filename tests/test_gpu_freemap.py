@@ -388,7 +388,9 @@ def test_reach_box_grazing_rays(ctx, n):
     (widened by the traversal's reach) end as misses before the walk: a slab
     seen from just above its top face at grazing angles, so that many lines
     pass the box by less than the widening and others by more.  Every output
-    but the step counts equals the oracle's."""
+    but the step counts equals the oracle's; the reference layout of the
+    same voxels (its box scanned from the block pool) gives the same hits."""
+    import brickmap_util as BU
     pal = S.palette()
     rng = np.random.default_rng(77 + n)
     g = np.zeros((n, n, n), np.uint8)
@@ -396,6 +398,7 @@ def test_reach_box_grazing_rays(ctx, n):
     g[n // 8:7 * n // 8, y0:y1, n // 8:7 * n // 8] = 4
     g[n // 2, y1:y1 + n // 8, n // 2] = 6  # a post on it
     wd = vt.World.dense(ctx, g, pal)
+    wb = vt.World.brickmap(ctx, *BU.embedding_a(g, pal))
     ow = O.DenseWorld(g, pal)
     W, H = 64, 48
     cap = 4 * n
@@ -406,16 +409,21 @@ def test_reach_box_grazing_rays(ctx, n):
         label = f"grazing n={n} camera {k}"
         log(label)
         compare(render_free(ctx, wd, cam, W, H, cap, cap, sun), o, stats=False, label=label)
+        gb = render_free(ctx, wb, cam, W, H, cap, cap, sun, gbuffer=False)
+        assert np.array_equal(gb["hits"], o["hits"]), f"{label} reference layout: hits"
+        assert np.max(np.abs(gb["shaded"] - o["shaded"])) <= 1e-5, f"{label} reference layout"
         hit_share.append(float(np.mean(o["hits"][..., 0] != 0xFFFFFFFF)))
     log(f"grazing n={n}: hit shares {[round(x, 2) for x in hit_share]}")
     assert 0.05 < np.mean(hit_share) < 0.95
     wd.close()
+    wb.close()
 
 
 def test_reach_box_grows_with_edits(ctx):
     """The box follows edits: voxels set far outside it after the world was
-    made are hit (rays toward them are not ended by the stale box), and a
-    world made empty and filled again by edits renders like the oracle."""
+    made are hit (rays toward them are not ended by the stale box), on both
+    layouts, and a world made empty and filled again by edits renders like
+    the oracle."""
     pal = S.palette()
     n = 128
     g = np.zeros((n, n, n), np.uint8)
@@ -445,3 +453,15 @@ def test_reach_box_grows_with_edits(ctx):
     compare(render_free(ctx, w, cam, W, H, 512, 512, S.SUN), o, stats=False,
             label="empty world, wall added")
     w.close()
+    # the reference layout: the small box's world, then the wall as block edits
+    import brickmap_util as BU
+    g = np.zeros((n, n, n), np.uint8)
+    g[10:20, 2:5, 10:20] = 3
+    wb = vt.World.brickmap(ctx, *BU.embedding_a(g, pal))
+    wb.set_voxels(xyz, np.full(len(xyz), 7, np.uint32))
+    g[80:82, 80:110, 40:90] = 7
+    o = O.render(O.DenseWorld(g, pal), cam, W, H, 512, 512, S.SUN)
+    gb = render_free(ctx, wb, cam, W, H, 512, 512, S.SUN, gbuffer=False)
+    assert np.array_equal(gb["hits"], o["hits"]), "reference layout, wall added: hits"
+    assert np.max(np.abs(gb["shaded"] - o["shaded"])) <= 1e-5
+    wb.close()

@@ -89,6 +89,12 @@ def test_procgen_world_renders(ctx, atlas):
         fh = f.hits.cpu().numpy().view(np.uint32)
         assert np.array_equal(fh, g.hits.cpu().numpy().view(np.uint32)), f"view {k}"
         assert np.array_equal(f.steps.cpu().numpy(), g.steps.cpu().numpy())
+        # rays ended early (free-region test, box of the generated blocks):
+        # the same hits
+        e = vt.Frame.alloc(W, H)
+        vt.render(ctx, w, cam, e, vt.params(W, H, 2048, 512, S.SUN, vt.VT_FLAG_FREE_MAP_ON))
+        torch.cuda.synchronize()
+        assert np.array_equal(fh, e.hits.cpu().numpy().view(np.uint32)), f"view {k} early end"
         rows = (80, 96)
         o = O.render(ow, cam, W, H, 2048, 512, S.SUN, rows=rows, gbuffer=False)
         assert np.array_equal(fh[rows[0]:rows[1]], o["hits"][rows[0]:rows[1]]), f"view {k}"

@@ -119,12 +119,51 @@ DenseWorld dense_view(const vt_world* w) {
   d.freemap = w->freemap;
   d.free_oct_bytes = free_words_per_octant(w->dim) * 4u;
   d.free_bytes = w->freemap && w->free_valid ? 8u * d.free_oct_bytes : 0u;
+  reach_of(w, d.reach_lo, d.reach_hi);
+  return d;
+}
+
+// The box of the nonempty voxels widened by wide_margin (the whole grid
+// while the world has none: no ray ends early for the box).
+void reach_of(const vt_world* w, float* lo, float* hi) {
   const float m = wide_margin(w->dim);
   for (int i = 0; i < 3; i++) {
-    d.reach_lo[i] = w->box_any ? (float)w->box_lo[i] - m : 0.0f;
-    d.reach_hi[i] = w->box_any ? (float)w->box_hi[i] + 1.0f + m : (float)w->dim;
+    lo[i] = w->box_any ? (float)w->box_lo[i] - m : 0.0f;
+    hi[i] = w->box_any ? (float)w->box_hi[i] + 1.0f + m : (float)w->dim;
   }
-  return d;
+}
+
+void grow_box(vt_world* w, int32_t x, int32_t y, int32_t z) {
+  const int32_t c[3] = {x, y, z};
+  for (int a = 0; a < 3; a++) {
+    w->box_lo[a] = w->box_any ? std::min(w->box_lo[a], c[a]) : c[a];
+    w->box_hi[a] = w->box_any ? std::max(w->box_hi[a], c[a]) : c[a];
+  }
+  w->box_any = true;
+}
+
+// The box of the nonzero blocks of a reference-layout map: exact from the
+// pool when `pool` is given, else the box of the nonzero chunk-table entries
+// (8^3 blocks each).
+void scan_brickmap_box(const uint32_t* chunks, const uint32_t* pool, uint32_t dim, vt_world* w) {
+  w->box_any = false;
+  const uint32_t M = dim / 8u;
+  for (uint32_t cz = 0; cz < M; cz++)
+    for (uint32_t cy = 0; cy < M; cy++)
+      for (uint32_t cx = 0; cx < M; cx++) {
+        const uint32_t e = chunks[cx + M * (cy + M * cz)];
+        if (!e) continue;
+        if (!pool) {
+          grow_box(w, (int32_t)(8u * cx), (int32_t)(8u * cy), (int32_t)(8u * cz));
+          grow_box(w, (int32_t)(8u * cx + 7u), (int32_t)(8u * cy + 7u), (int32_t)(8u * cz + 7u));
+          continue;
+        }
+        const uint32_t* b = pool + (size_t)(e - 1u) * 512u;  // in-block x + 8y + 64z
+        for (uint32_t i = 0; i < 512u; i++)
+          if (b[i])
+            grow_box(w, (int32_t)(8u * cx + (i & 7u)), (int32_t)(8u * cy + ((i >> 3) & 7u)),
+                     (int32_t)(8u * cz + (i >> 6)));
+      }
 }
 
 // The box of the nonempty voxels of a linear grid (x + dim (y + dim z)).
@@ -175,6 +214,7 @@ BrickmapWorld brickmap_view(const vt_world* w) {
   b.freemap = w->freemap;
   b.free_oct_bytes = free_words_per_octant(w->dim) * 4u;
   b.free_bytes = w->freemap && w->free_valid ? 8u * b.free_oct_bytes : 0u;
+  reach_of(w, b.reach_lo, b.reach_hi);
   return b;
 }
 
@@ -480,6 +520,7 @@ int vt_world_create_brickmap(vt_ctx* ctx, uint32_t dim, const uint32_t* chunks,
     return fail(VT_ERR_OUT_OF_MEMORY, "vt_world_create_brickmap: host chunk table");
   }
   std::memcpy(w->chunk_mirror, chunks, nchunks * 4);
+  vt::scan_brickmap_box(chunks, pool, dim, w);
   hipError_t e = hipMemcpy(w->chunks, chunks, nchunks * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess && pool_words)
     e = hipMemcpy(w->pool, pool, pool_words * 4, hipMemcpyHostToDevice);

@@ -251,6 +251,15 @@ int check_coords(const vt_world* w, const uint32_t* xyz, uint32_t n, const char*
   return VT_OK;
 }
 
+// The box of the nonempty voxels grows with every voxel set nonzero (a
+// cleared voxel leaves it as it was: still a box around all of them).
+static void grow_box_by(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint32_t n) {
+  for (uint32_t i = 0; i < n; i++)
+    if (values[i] != 0u)
+      vt::grow_box(w, (int32_t)xyz[3 * (size_t)i], (int32_t)xyz[3 * (size_t)i + 1],
+                   (int32_t)xyz[3 * (size_t)i + 2]);
+}
+
 int set_brickmap(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint32_t n,
                  hipStream_t s) {
   const uint32_t M = w->dim / 8u;
@@ -264,6 +273,7 @@ int set_brickmap(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint3
   std::unordered_map<uint64_t, uint32_t> fresh;
   uint64_t used = w->pool_words / 512u;
   last.reserve(n);
+  grow_box_by(w, xyz, values, n);
   for (uint32_t i = 0; i < n; i++) {
     const uint32_t x = xyz[3 * (size_t)i], y = xyz[3 * (size_t)i + 1], z = xyz[3 * (size_t)i + 2];
     const uint64_t ci = (uint64_t)(x / 8u) + M * ((uint64_t)(y / 8u) + (uint64_t)M * (z / 8u));
@@ -348,17 +358,7 @@ int set_dense(vt_world* w, const uint32_t* xyz, const uint32_t* values, uint32_t
   std::vector<uint64_t> addr;
   std::vector<uint32_t> val, bricks, regions, regions8;
   last.reserve(n);
-  // the box of the nonempty voxels grows with every voxel set nonzero (a
-  // cleared voxel leaves it as it was: still a box around all of them)
-  for (uint32_t i = 0; i < n; i++) {
-    if (values[i] == 0u) continue;
-    for (int a = 0; a < 3; a++) {
-      const int32_t c = (int32_t)xyz[3 * (size_t)i + a];
-      w->box_lo[a] = w->box_any ? std::min(w->box_lo[a], c) : c;
-      w->box_hi[a] = w->box_any ? std::max(w->box_hi[a], c) : c;
-    }
-    w->box_any = true;
-  }
+  grow_box_by(w, xyz, values, n);
   for (uint32_t i = 0; i < n; i++) {
     const uint32_t code =
         morton_code(xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2]);

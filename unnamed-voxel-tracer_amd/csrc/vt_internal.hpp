@@ -51,8 +51,9 @@ struct vt_world {
   uint8_t* free_tmp;       // 8 x free_regions(dim)^3 bytes of build scratch
   bool free_valid;         // the map matches the occupancy (finish_freemap)
   float free_frac;         // largest share of free regions over the 8 octants
-  // dense: the box of the nonempty voxels (inclusive; grown by edits, never
-  // shrunk), box_any = false while the world has none
+  // the box of the nonempty voxels (dense) or blocks (reference layout),
+  // inclusive; grown by edits, never shrunk; box_any = false while the
+  // world has none
   int32_t box_lo[3], box_hi[3];
   bool box_any;
   // compact albedo index of a dense world (built on request)
@@ -110,6 +111,9 @@ struct TileShape {
 
 DenseWorld dense_view(const vt_world* w);
 void scan_dense_box(const uint8_t* grid, uint32_t dim, vt_world* w);
+void scan_brickmap_box(const uint32_t* chunks, const uint32_t* pool, uint32_t dim, vt_world* w);
+void grow_box(vt_world* w, int32_t x, int32_t y, int32_t z);
+void reach_of(const vt_world* w, float* lo, float* hi);
 BrickmapWorld brickmap_view(const vt_world* w);
 
 // Takes the device address of the no-allocation scratch array on `device`

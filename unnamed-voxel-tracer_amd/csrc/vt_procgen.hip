@@ -412,6 +412,7 @@ int vt_world_procgen(vt_ctx* ctx, vt_world* world, float offset_x, float offset_
   }
   std::memcpy(world->chunk_mirror, walk.mirror.data(), nchunks * 4);
   world->pool_words = walk.used * 512u;
+  vt::scan_brickmap_box(world->chunk_mirror, nullptr, dim, world);  // chunk granularity
   VT_HIP(vt::finish_freemap(world));
   return VT_OK;
 }

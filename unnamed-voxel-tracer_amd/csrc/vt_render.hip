@@ -96,7 +96,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     float ey = (v.pos[1] + dy * tc) - 0.001f;
     float ez = (v.pos[2] + dz * tc) - 0.001f;
     GeneralHit g;
-    if (!kStats && w.occ)
+    if (kFree && !kStats && !line_meets_reach_bm(w, ex, ey, ez, dx, dy, dz)) {
+      // with the free-region test rays end early: this one cannot reach a
+      // nonempty block (a miss, no steps)
+      g.data = 0u;
+      g.hx = g.hy = g.hz = -1.0f;
+      g.steps = 0;
+    } else if (!kStats && w.occ)
       trace_brickmap_fast<false, kFree>(w, ex, ey, ez, dx, dy, dz, sc.max_primary, g);
     else
       trace_general<kStats>(w, ex, ey, ez, dx, dy, dz, sc.max_primary, g, cnt.l1f, cnt.l0f);

@@ -444,17 +444,23 @@ __device__ __forceinline__ void dense_ray_consts(DenseRay& r, const DenseWorld& 
 __host__ __device__ __forceinline__ float wide_margin(uint32_t dim) {
   return 2.5f + (float)dim / 1000.0f;
 }
-__device__ __forceinline__ bool line_meets_reach(const DenseWorld& w, float ex, float ey,
-                                                 float ez, const RayConsts& k) {
+// The slab test of the line against [lo, hi] (inv = 1 / d after the zero
+// substitution, so never infinite).
+__device__ __forceinline__ bool line_meets_box(const float* lo, const float* hi, float ex,
+                                               float ey, float ez, const float* inv) {
   const float e[3] = {ex, ey, ez};
   float tn = 0.0f, tf = 3.0e38f;
 #pragma unroll
   for (int i = 0; i < 3; i++) {
-    const float a = (w.reach_lo[i] - e[i]) * k.inv[i], b = (w.reach_hi[i] - e[i]) * k.inv[i];
+    const float a = (lo[i] - e[i]) * inv[i], b = (hi[i] - e[i]) * inv[i];
     tn = fmaxf(tn, fminf(a, b));
     tf = fminf(tf, fmaxf(a, b));
   }
   return tn <= tf;
+}
+__device__ __forceinline__ bool line_meets_reach(const DenseWorld& w, float ex, float ey,
+                                                 float ez, const RayConsts& k) {
+  return line_meets_box(w.reach_lo, w.reach_hi, ex, ey, ez, k.inv);
 }
 
 // Set-up (map.glsl:85-104) plus iteration 0 (stepSize 0).  Returns true when
@@ -1024,7 +1030,20 @@ struct BrickmapWorld {
   // free-region map of the occupancy bytes; free_bytes 0 = none
   const uint32_t* freemap;
   uint32_t free_bytes, free_oct_bytes;
+  // the box of the nonempty blocks widened by the traversal's reach (as
+  // DenseWorld's: line_meets_reach)
+  float reach_lo[3], reach_hi[3];
 };
+
+// line_meets_reach of a reference-layout world: the same exactness argument
+// (every probe before the first nonempty block is a block-level step of the
+// same recurrence); d is the direction before the zero substitution.
+__device__ __forceinline__ bool line_meets_reach_bm(const BrickmapWorld& w, float ex, float ey,
+                                                    float ez, float dx, float dy, float dz) {
+  const float inv[3] = {1.0f / (dx == 0.0f ? 0.001f : dx), 1.0f / (dy == 0.0f ? 0.001f : dy),
+                        1.0f / (dz == 0.0f ? 0.001f : dz)};
+  return line_meets_box(w.reach_lo, w.reach_hi, ex, ey, ez, inv);
+}
 
 // map_getVoxel (map.glsl:38-47) of block coordinates b; counts the chunk-table
 // and pool reads it makes.
