@@ -43,10 +43,10 @@ def boxes_world(n, seed, n_boxes=6, n_single=12):
     return g
 
 
-def render_free(ctx, world, cam, W, H, cap_p, cap_s, sun, flags=ON, gbuffer=True):
+def render_free(ctx, world, cam, W, H, cap_p, cap_s, sun, flags=ON, gbuffer=True, traversal=0):
     import torch
     f = poison(vt.Frame.alloc(W, H, gbuffer=gbuffer))
-    vt.render(ctx, world, cam, f, vt.params(W, H, cap_p, cap_s, sun, flags))
+    vt.render(ctx, world, cam, f, vt.params(W, H, cap_p, cap_s, sun, flags, traversal))
     torch.cuda.synchronize()
     out = {"shaded": f.shaded.cpu().numpy(), "hits": f.hits.cpu().numpy().view(np.uint32)}
     if gbuffer:
@@ -388,8 +388,10 @@ def test_reach_box_grazing_rays(ctx, n):
     (widened by the traversal's reach) end as misses before the walk: a slab
     seen from just above its top face at grazing angles, so that many lines
     pass the box by less than the widening and others by more.  Every output
-    but the step counts equals the oracle's; the reference layout of the
-    same voxels (its box scanned from the block pool) gives the same hits."""
+    but the step counts equals the oracle's, with the free-region test and
+    without it (the box is taken whenever a frame stores no step counts);
+    the reference layout of the same voxels (its box scanned from the block
+    pool) gives the same hits."""
     import brickmap_util as BU
     pal = S.palette()
     rng = np.random.default_rng(77 + n)
@@ -409,6 +411,11 @@ def test_reach_box_grazing_rays(ctx, n):
         label = f"grazing n={n} camera {k}"
         log(label)
         compare(render_free(ctx, wd, cam, W, H, cap, cap, sun), o, stats=False, label=label)
+        # the box without the free-region test (frames that store no step
+        # counts), flat and bitmask traversals
+        for trav in (vt.VT_TRAVERSAL_FLAT, vt.VT_TRAVERSAL_BITMASK):
+            compare(render_free(ctx, wd, cam, W, H, cap, cap, sun, flags=OFF, traversal=trav), o,
+                    stats=False, label=f"{label} test off, traversal {trav}")
         gb = render_free(ctx, wb, cam, W, H, cap, cap, sun, gbuffer=False)
         assert np.array_equal(gb["hits"], o["hits"]), f"{label} reference layout: hits"
         assert np.max(np.abs(gb["shaded"] - o["shaded"])) <= 1e-5, f"{label} reference layout"

@@ -175,6 +175,8 @@ hipError_t launch_batch(const vt_world* world, const vt::ShadeConsts& sc_in,
                         unsigned long long* stats, bool steps_out, hipStream_t s) {
   vt::DenseWorld w = vt::dense_view(world);
   if (steps_out || stats || !vt::use_free_map(world, sc_in)) w.free_bytes = 0;
+  // rays end early at the reach box unless their step counts are observed
+  if (steps_out || stats) vt::reach_everywhere(w.reach_lo, w.reach_hi);
   vt::ShadeConsts sc = sc_in;
   sc.shadow = vt::make_ray_consts(sc.sun[0], sc.sun[1], sc.sun[2], w.mx, w.my, w.mz);
   hipLaunchKernelGGL(vt::k_batch_reset, dim3(1), dim3(64), 0, s, d_queues);

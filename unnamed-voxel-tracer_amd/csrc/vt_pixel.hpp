@@ -653,9 +653,9 @@ __device__ __forceinline__ void trace_pixel_dense(const DenseWorld& w, const Vie
   float ey = (v.pos[1] + dy * tc) - 0.001f;
   float ez = (v.pos[2] + dz * tc) - 0.001f;
   const RayConsts pk = make_ray_consts(dx, dy, dz, w.mx, w.my, w.mz);
-  if (L::kFree && !line_meets_reach(w, ex, ey, ez, pk)) {
-    // with the free-region test rays end early: this one cannot reach a
-    // nonempty voxel (a miss, no steps)
+  if (!line_meets_reach(w, ex, ey, ez, pk)) {
+    // this ray cannot reach a nonempty voxel: a miss, no steps (launches
+    // whose step counts are observed pass a reach every line meets)
     h.hit = false;
     h.steps = 0;
   } else if (!dense_begin_k<kStats>(r, lk, w, ex, ey, ez, pk, sc.max_primary, h, cnt.l1f,

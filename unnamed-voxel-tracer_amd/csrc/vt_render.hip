@@ -97,8 +97,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     float ez = (v.pos[2] + dz * tc) - 0.001f;
     GeneralHit g;
     if (kFree && !kStats && !line_meets_reach_bm(w, ex, ey, ez, dx, dy, dz)) {
-      // with the free-region test rays end early: this one cannot reach a
-      // nonempty block (a miss, no steps)
+      // with the free-region test: this ray cannot reach a nonempty block (a
+      // miss, no steps).  Without the test the check is left out: the
+      // engine's own frame (caps 192/48, its camera inside the box, where
+      // no ray can end) measured 0.304 ms with it against 0.274 without.
       g.data = 0u;
       g.hx = g.hy = g.hz = -1.0f;
       g.steps = 0;
@@ -232,6 +234,9 @@ hipError_t launch_render_dense(const DenseWorld& w_in, uint32_t traversal, const
   (void)compute_units;
   DenseWorld w = w_in;
   if (out.steps) w.free_bytes = 0;  // step counts are stored: no early end of rays
+  // nor where they are stored, or counted as traceMap's (statistics without
+  // the free-region test), for the reach box
+  if (out.steps || (out.stats && !w.free_bytes)) reach_everywhere(w.reach_lo, w.reach_hi);
   ShadeConsts sc = sc_in;
   sc.shadow = make_ray_consts(sc.sun[0], sc.sun[1], sc.sun[2], w.mx, w.my, w.mz);
   uint32_t blocks_x, nblocks;
@@ -266,7 +271,10 @@ hipError_t launch_render_brickmap(const BrickmapWorld& w_in, const ViewConsts& v
   BrickmapWorld w = w_in;
   // step counts stored, or statistics (of traceMap's steps on this layout):
   // no early end of rays
-  if (out.steps || out.stats) w.free_bytes = 0;
+  if (out.steps || out.stats) {
+    w.free_bytes = 0;
+    reach_everywhere(w.reach_lo, w.reach_hi);
+  }
   uint32_t blocks_x, nblocks;
   grid_shape(v, tile, blocks_x, nblocks);
   if (schedule && nblocks > VT_SCHEDULE_BLOCKS(v.W, v.H)) return hipErrorInvalidValue;

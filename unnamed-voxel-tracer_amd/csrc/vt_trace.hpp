@@ -211,7 +211,8 @@ struct DenseWorld {
   const uint32_t* freemap;
   uint32_t free_bytes, free_oct_bytes;
   // the box of the nonempty voxels widened by the traversal's reach
-  // (line_meets_reach); the whole grid when the world is empty
+  // (line_meets_reach); the whole grid when the world is empty; everywhere
+  // (reach_everywhere) when step counts are observed
   float reach_lo[3], reach_hi[3];
 };
 
@@ -443,6 +444,15 @@ __device__ __forceinline__ void dense_ray_consts(DenseRay& r, const DenseWorld& 
 // as a miss; only its step count differs from traceMap's.
 __host__ __device__ __forceinline__ float wide_margin(uint32_t dim) {
   return 2.5f + (float)dim / 1000.0f;
+}
+// A reach every line meets: for launches whose step counts are observed
+// (stored per pixel, or statistics of traceMap's steps), where no ray may
+// end before traceMap does.
+__host__ __device__ inline void reach_everywhere(float* lo, float* hi) {
+  for (int i = 0; i < 3; i++) {
+    lo[i] = -1.0e30f;
+    hi[i] = 1.0e30f;
+  }
 }
 // The slab test of the line against [lo, hi] (inv = 1 / d after the zero
 // substitution, so never infinite).
