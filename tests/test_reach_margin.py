@@ -124,3 +124,65 @@ def test_needed_widening_closed_forms():
     # block (3, 0): the line x - y = 0 against [3, 4] x [0, 1]: distance
     # min over t of max(3 - x, y - 1) with x = y -> 1 at x = y = 2
     assert np.allclose(w2, [1.0, 0.0])
+
+
+def blind_steps_f32(e, d, lo, hi):
+    """The number of leading steps whose lookups the traversal skips (the
+    kernel's blind_steps, binary32): the block boundaries the line crosses
+    before it enters the reach box [lo, hi], less 13; None when the line
+    misses the reach box (the ray ends before its walk)."""
+    e = e.astype(np.float32)
+    d = np.where(d == 0.0, np.float32(0.001), d).astype(np.float32)
+    inv = (np.float32(1.0) / d).astype(np.float32)
+    a = ((lo.astype(np.float32) - e) * inv).astype(np.float32)
+    b = ((hi.astype(np.float32) - e) * inv).astype(np.float32)
+    tn = np.float32(max(0.0, float(np.max(np.minimum(a, b)))))
+    if tn > np.float32(min(3.0e38, float(np.min(np.maximum(a, b))))):
+        return None
+    p = (e + d * tn).astype(np.float32)
+    return int(np.sum(np.abs(np.floor(p).astype(np.int64) - np.floor(e).astype(np.int64)))) - 13
+
+
+@pytest.mark.parametrize("dim", [64, 256, 1024])
+def test_blind_steps_probe_outside_the_box(dim):
+    """The steps a primary ray takes before its lookups are made (toward the
+    box of the nonempty voxels from outside its reach) probe only blocks
+    outside that box: the oracle's walk, probe by probe, against boxes of
+    every size, with rays aimed at the box, grazing it and crossing it at
+    every angle.  Also reports how many probes the bound leaves to the
+    lookups before the first one inside the box."""
+    grid = np.zeros((dim, dim, dim), np.uint8)
+    world = O.DenseWorld(grid, np.zeros(256, np.uint32))
+    rng = np.random.default_rng(3 * dim)
+    m = wide_margin(dim)
+    slack_min, checked = 10 ** 9, 0
+    for k in range(300):
+        size = rng.integers(1, dim // 2, 3)
+        blo = rng.integers(0, dim - size + 1)
+        bhi = blo + size - 1  # inclusive block box
+        lo, hi = blo - m, bhi + 1.0 + m
+        e = rng.uniform(0.0, dim, 3)
+        if np.all((e >= lo) & (e <= hi)):
+            continue
+        tgt = rng.uniform(blo - 3.0, bhi + 4.0)
+        d = tgt - e
+        if k % 5 == 0:
+            d[rng.integers(0, 3)] = 0.0
+        d = (d / np.linalg.norm(d)).astype(np.float32)
+        e = e.astype(np.float32)
+        nb = blind_steps_f32(e, d, lo, hi)
+        if nb is None or nb <= 0:
+            continue
+        cap = 4 * dim
+        h, path = O.trace(world, e, d, cap, path_cap=cap)
+        blocks = path >> 3
+        inside = np.all((blocks >= blo) & (blocks <= bhi), axis=1)
+        first = int(np.argmax(inside)) if inside.any() else len(blocks)
+        assert first > nb, (f"dim {dim} ray {k}: probe {first} is inside the box but the "
+                            f"first {nb} steps skip their lookups")
+        if inside.any():
+            slack_min = min(slack_min, first - nb)
+        checked += 1
+    print(f"[blind] dim {dim}: {checked} rays; fewest probes between the skipped steps and "
+          f"the first probe inside the box: {slack_min}")
+    assert checked > 100

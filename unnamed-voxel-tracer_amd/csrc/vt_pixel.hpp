@@ -660,6 +660,7 @@ __device__ __forceinline__ void trace_pixel_dense(const DenseWorld& w, const Vie
     h.steps = 0;
   } else if (!dense_begin_k<kStats>(r, lk, w, ex, ey, ez, pk, sc.max_primary, h, cnt.l1f,
                                     cnt.l0f)) {
+    r.blind = blind_steps(w, ex, ey, ez, pk);
     dense_run<kStats>(r, lk, w, h, cnt.l1f, cnt.l0f);
   }
   p.hit = h.hit;

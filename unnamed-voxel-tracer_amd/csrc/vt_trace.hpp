@@ -368,6 +368,7 @@ struct DenseRay {
                       // travels negatively stored complemented
   uint32_t flip;      // the complemented axes' masks; address = c ^ flip
   int steps, max_steps;
+  int blind;          // steps whose lookups can only read empty (blind_steps)
 };
 
 // The per-ray constants of traceMap's set-up (map.glsl:85-97): the 0.001
@@ -473,6 +474,32 @@ __device__ __forceinline__ bool line_meets_reach(const DenseWorld& w, float ex, 
   return line_meets_box(w.reach_lo, w.reach_hi, ex, ey, ez, k.inv);
 }
 
+// How many of traceMap's steps from e (voxel units) along k.d, whose line
+// meets the reach box, come before the walk can probe a nonempty voxel: the
+// blocks the line crosses before it enters the reach box, less a slack.
+// Why the lookups of the first blind_steps steps read empty: after k steps
+// the walk has crossed k block boundaries, c_i of them on axis i, and its
+// position lies within the drift (< 1.03 voxel, see above) of a point
+// L(s) of the line; the boundaries the line crosses up to s, n_i(s), differ
+// from c_i by at most 4 (2 for the drift, 1 for a within that reached 8, 1
+// for the binary32 floor below), so k <= n(t_enter) - 13 gives n(s) <
+// n(t_enter), s < t_enter, L(s) outside the reach box, and the probe
+// outside the box of the nonempty voxels (the reach box's argument).
+__device__ __forceinline__ int blind_steps(const DenseWorld& w, float ex, float ey, float ez,
+                                           const RayConsts& k) {
+  const float e[3] = {ex, ey, ez};
+  float tn = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const float a = (w.reach_lo[i] - e[i]) * k.inv[i], b = (w.reach_hi[i] - e[i]) * k.inv[i];
+    tn = fmaxf(tn, fminf(a, b));
+  }
+  int n = 0;
+#pragma unroll
+  for (int i = 0; i < 3; i++) n += abs((int)floorf(e[i] + k.d[i] * tn) - (int)floorf(e[i]));
+  return n - 13;
+}
+
 // Set-up (map.glsl:85-104) plus iteration 0 (stepSize 0).  Returns true when
 // the ray already terminated (hit or miss) in iteration 0; otherwise the ray
 // is at the block level, ready for the DDA step of iteration 0.
@@ -500,6 +527,7 @@ __device__ __forceinline__ bool dense_begin_k(DenseRay& r, L& lk, const DenseWor
   dense_ray_from(r, k);
   r.steps = 0;
   r.max_steps = max_steps;
+  r.blind = 0;
   lk.reset();
 
   float s8x = ox * 8.0f, s8y = oy * 8.0f, s8z = oz * 8.0f;
@@ -730,6 +758,45 @@ __device__ __forceinline__ bool free_bit(uint32_t word, uint32_t g) {
   return ((word >> ((g >> 12) & 31u)) & 1u) != 0u;
 }
 
+// Steps K at a time from s while steps + K <= lim without their lookups:
+// the caller guarantees that they all read empty (blind_steps).  Stops
+// before an iteration that leaves the grid; the state is traceMap's after
+// `steps` steps, for the empty-space phase to continue from.
+template <int K>
+__device__ __forceinline__ void dense_skip_blind(float Px, float Py, float Pz, float Ax,
+                                                 float Ay, float Az, float dx, float dy, float dz,
+                                                 float ix, float iy, float iz, uint32_t mx,
+                                                 uint32_t my, uint32_t mz, uint32_t flip,
+                                                 DenseCell& s, int& steps, int lim) {
+  float bx = s.wx, by = s.wy, bz = s.wz;
+  uint32_t bc = s.c;
+  int sb = steps;
+  while (sb + K <= lim) {
+    DenseCell n;
+    n.wx = bx;
+    n.wy = by;
+    n.wz = bz;
+    n.c = bc;
+    bool wrapped = false;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      n = dense_step(Px, Py, Pz, Ax, Ay, Az, dx, dy, dz, ix, iy, iz, mx, my, mz, flip, n);
+      wrapped |= n.wrapped;
+    }
+    if (wrapped) break;
+    bx = n.wx;
+    by = n.wy;
+    bz = n.wz;
+    bc = n.c;
+    sb += K;
+  }
+  s.wx = bx;
+  s.wy = by;
+  s.wz = bz;
+  s.c = bc;
+  steps = sb;
+}
+
 // Empty-space phase of the flat loop.  An iteration takes K block-level steps
 // from `s` (whose own lookup was empty, `steps` steps done) and requests all K
 // bytes without testing any of them in between; only when all K are zero and
@@ -913,10 +980,17 @@ __device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const Den
   if constexpr (kPositive)
     status = dense_skip_positive<L::kSkip, L::kFree>(r.dx, r.dy, r.dz, r.ix, r.iy, r.iz, mx,
                                                           my, mz, s0, steps0, max_steps, vox, fm);
-  else
+  else {
+    // the lookups that can only read empty first (primary rays), the
+    // iteration that reaches the cap left to the loops below
+    if (r.blind > steps0)
+      dense_skip_blind<L::kSkip>(r.Px, r.Py, r.Pz, r.Ax, r.Ay, r.Az, r.dx, r.dy, r.dz, r.ix,
+                                 r.iy, r.iz, mx, my, mz, flip, s0, steps0,
+                                 min(r.blind, max_steps - 1));
     status = dense_skip_empty<L::kSkip, L::kFree>(
         r.Px, r.Py, r.Pz, r.Ax, r.Ay, r.Az, r.dx, r.dy, r.dz, r.ix, r.iy, r.iz, mx, my, mz, flip,
         s0, steps0, max_steps, vox, fm, free_octant_offset(flip, mx, my, mz, w.free_oct_bytes));
+  }
   if (L::kFree && status == kSkipFree) {  // a miss: no later lookup can be nonzero
     h.hit = false;
     h.steps = steps0 + L::kSkip;  // the steps executed, the last iteration's included
