@@ -96,7 +96,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     float ey = (v.pos[1] + dy * tc) - 0.001f;
     float ez = (v.pos[2] + dz * tc) - 0.001f;
     GeneralHit g;
-    if (kFree && !kStats && !line_meets_reach_bm(w, ex, ey, ez, dx, dy, dz)) {
+    int blind = 0;
+    if (kFree && !kStats && !line_meets_reach_bm(w, ex, ey, ez, dx, dy, dz, blind)) {
       // with the free-region test: this ray cannot reach a nonempty block (a
       // miss, no steps).  Without the test the check is left out: the
       // engine's own frame (caps 192/48, its camera inside the box, where
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
       g.hx = g.hy = g.hz = -1.0f;
       g.steps = 0;
     } else if (!kStats && w.occ)
-      trace_brickmap_fast<false, kFree>(w, ex, ey, ez, dx, dy, dz, sc.max_primary, g);
+      trace_brickmap_fast<false, kFree>(w, ex, ey, ez, dx, dy, dz, sc.max_primary, g, blind);
     else
       trace_general<kStats>(w, ex, ey, ez, dx, dy, dz, sc.max_primary, g, cnt.l1f, cnt.l0f);
     Primary p;

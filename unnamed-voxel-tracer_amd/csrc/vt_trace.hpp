@@ -485,19 +485,24 @@ __device__ __forceinline__ bool line_meets_reach(const DenseWorld& w, float ex, 
 // for the binary32 floor below), so k <= n(t_enter) - 13 gives n(s) <
 // n(t_enter), s < t_enter, L(s) outside the reach box, and the probe
 // outside the box of the nonempty voxels (the reach box's argument).
-__device__ __forceinline__ int blind_steps(const DenseWorld& w, float ex, float ey, float ez,
-                                           const RayConsts& k) {
+__device__ __forceinline__ int blind_steps_box(const float* lo, const float* hi, float ex,
+                                               float ey, float ez, const float* d,
+                                               const float* inv) {
   const float e[3] = {ex, ey, ez};
   float tn = 0.0f;
 #pragma unroll
   for (int i = 0; i < 3; i++) {
-    const float a = (w.reach_lo[i] - e[i]) * k.inv[i], b = (w.reach_hi[i] - e[i]) * k.inv[i];
+    const float a = (lo[i] - e[i]) * inv[i], b = (hi[i] - e[i]) * inv[i];
     tn = fmaxf(tn, fminf(a, b));
   }
   int n = 0;
 #pragma unroll
-  for (int i = 0; i < 3; i++) n += abs((int)floorf(e[i] + k.d[i] * tn) - (int)floorf(e[i]));
+  for (int i = 0; i < 3; i++) n += abs((int)floorf(e[i] + d[i] * tn) - (int)floorf(e[i]));
   return n - 13;
+}
+__device__ __forceinline__ int blind_steps(const DenseWorld& w, float ex, float ey, float ez,
+                                           const RayConsts& k) {
+  return blind_steps_box(w.reach_lo, w.reach_hi, ex, ey, ez, k.d, k.inv);
 }
 
 // Set-up (map.glsl:85-104) plus iteration 0 (stepSize 0).  Returns true when
@@ -1121,11 +1126,15 @@ struct BrickmapWorld {
 
 // line_meets_reach of a reference-layout world: the same exactness argument
 // (every probe before the first nonempty block is a block-level step of the
-// same recurrence); d is the direction before the zero substitution.
+// same recurrence); d is the direction before the zero substitution.  Also
+// the ray's blind_steps.
 __device__ __forceinline__ bool line_meets_reach_bm(const BrickmapWorld& w, float ex, float ey,
-                                                    float ez, float dx, float dy, float dz) {
-  const float inv[3] = {1.0f / (dx == 0.0f ? 0.001f : dx), 1.0f / (dy == 0.0f ? 0.001f : dy),
-                        1.0f / (dz == 0.0f ? 0.001f : dz)};
+                                                    float ez, float dx, float dy, float dz,
+                                                    int& blind) {
+  const float d[3] = {dx == 0.0f ? 0.001f : dx, dy == 0.0f ? 0.001f : dy,
+                      dz == 0.0f ? 0.001f : dz};
+  const float inv[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
+  blind = blind_steps_box(w.reach_lo, w.reach_hi, ex, ey, ez, d, inv);
   return line_meets_box(w.reach_lo, w.reach_hi, ex, ey, ez, inv);
 }
 
@@ -1266,9 +1275,12 @@ __device__ bool trace_general(const BrickmapWorld& w, float ox, float oy, float 
 // kFree: the empty-space loop tests the free-region map of the occupancy
 // bytes and ends a ray that can hit nothing more; the launcher takes it for
 // long step caps, so it also takes the longer empty-space iteration.
+// blind: leading steps whose lookups can only read empty (blind_steps; 0
+// for none).
 template <bool kPositive = false, bool kFree = false>
 __device__ bool trace_brickmap_fast(const BrickmapWorld& w, float ox, float oy, float oz, float dx,
-                                    float dy, float dz, int max_steps, GeneralHit& h) {
+                                    float dy, float dz, int max_steps, GeneralHit& h,
+                                    int blind = 0) {
   constexpr int kSkip = kFree ? kSkipLong : kSkipShort;
   if (dx == 0.0f) dx = 0.001f;
   if (dy == 0.0f) dy = 0.001f;
@@ -1379,11 +1391,16 @@ __device__ bool trace_brickmap_fast(const BrickmapWorld& w, float ox, float oy, 
     if constexpr (kPositive)
       status = dense_skip_positive<kSkip, kFree>(d[0], d[1], d[2], inv[0], inv[1], inv[2],
                                                          mx, my, mz, s0, s0steps, lim, occ, fm);
-    else
+    else {
+      if (blind > s0steps)
+        dense_skip_blind<kSkip>(P3[0], P3[1], P3[2], A3[0], A3[1], A3[2], d[0], d[1], d[2],
+                                inv[0], inv[1], inv[2], mx, my, mz, flip, s0, s0steps,
+                                min(blind, max_steps - 1));
       status = dense_skip_empty<kSkip, kFree>(
           P3[0], P3[1], P3[2], A3[0], A3[1], A3[2], d[0], d[1], d[2], inv[0], inv[1], inv[2], mx,
           my, mz, flip, s0, s0steps, lim, occ, fm,
           free_octant_offset(flip, mx, my, mz, w.free_oct_bytes));
+    }
     if (kFree && status == kSkipFree) {
       step = s0steps;
       break;
