@@ -9,8 +9,8 @@ precede the 200 timed ones.
 
 Prints ONE JSON line.  value = (primary + shadow rays) / kernel time, timed
 with HIP events on the stream the kernel is launched on.  roofline.achieved
-counts the algorithmic bytes of the launch (per executed lookup step the
-traversal's gathers, plus 28 B of shaded + hit-record output per pixel).
+counts the algorithmic bytes of the launch (per executed step with a lookup
+the traversal's gathers, plus 28 B of shaded + hit-record output per pixel).
 cpu_baseline = the oracle (CPU restatement) on the host cores, one frame.
 """
 import argparse
@@ -34,17 +34,26 @@ TRAVERSALS = ["flat", "bitmask", "bitmask8"]  # VT_TRAVERSAL_FLAT, _BITMASK, _BI
 OUT_BYTES_PER_PIXEL = 16 + 12  # RGBA32F shaded + 12-byte hit record
 
 
+def lookup_steps(stats):
+    """Executed steps of the flat traversal that made their lookup: all of
+    them, less those a primary ray takes before its line can reach the box of
+    the nonempty voxels (counted in the l1_fetches slot by the statistics of
+    executed steps; 0 in traceMap's)."""
+    return stats["primary_steps"] + stats["shadow_steps"] - stats["l1_fetches"]
+
+
 def algorithmic_bytes(stats, traversal, npix):
     """Bytes one launch must move: the traversal's gathers plus the outputs.
 
-    flat:    1 byte per executed traceMap step (the voxel gather);
+    flat:    1 byte per executed traceMap step with a lookup (the voxel
+             gather);
     bitmask, bitmask8: 8 B per region word + 8 B per leaf word fetched,
              + 1 B material per primary hit.
     """
     if traversal in ("bitmask", "bitmask8"):
         b = 8 * stats["l1_fetches"] + 8 * stats["l0_fetches"] + stats["leaf_reads"]
     else:
-        b = stats["primary_steps"] + stats["shadow_steps"]
+        b = lookup_steps(stats)
     return b + OUT_BYTES_PER_PIXEL * npix
 
 
@@ -165,7 +174,7 @@ def bench_batched(args):
     ms = timed(torch, stream, batch.graph_launch, args.steps, args.warmup)
     if TRAVERSALS[best.traversal] != "flat":
         ex = st  # the bitmask traversals run every ray to its end
-    abytes = ex["primary_steps"] + ex["shadow_steps"] + OUT_BYTES_PER_PIXEL * npix
+    abytes = lookup_steps(ex) + OUT_BYTES_PER_PIXEL * npix
     achieved = abytes / (ms * 1e-3) / 1e9
     bkey = ("k_render_batch/flat_free/1024cube_64views"
             if free and TRAVERSALS[best.traversal] == "flat" else None)
@@ -191,6 +200,7 @@ def bench_batched(args):
             "dda_steps": st["primary_steps"] + st["shadow_steps"],
             "free_region_test": free and TRAVERSALS[best.traversal] == "flat",
             "executed_steps": ex["primary_steps"] + ex["shadow_steps"],
+            "lookup_steps": lookup_steps(ex),
             "tuning": {"tile": {3: "8x8", 4: "16x4", 5: "32x2"}[best.tile],
                        "chunk": best.chunk,
                        "traversal": TRAVERSALS[best.traversal],
@@ -265,8 +275,7 @@ def bench_brickmap(args):
     # (an occupancy byte in the empty-space loop, a 4-byte chunk word in the
     # literal loop) + the outputs; the literal loop's own reads -- 4 B chunk,
     # 4 B pool, 4 B atlas per step -- are the upper figure
-    abytes = LOWER_BYTES_PER_STEP * (ex["primary_steps"] + ex["shadow_steps"]) + \
-        OUT_BYTES_PER_PIXEL * npix
+    abytes = LOWER_BYTES_PER_STEP * lookup_steps(ex) + OUT_BYTES_PER_PIXEL * npix
     achieved = abytes / (ms * 1e-3) / 1e9
     line = {
         "metric": "Mrays/s primary+shadow, 1080p/512³ grid in the engine's chunk/pool/atlas "
@@ -292,12 +301,13 @@ def bench_brickmap(args):
             "dda_steps": st["primary_steps"] + st["shadow_steps"],
             "free_region_test": free,
             "executed_steps": ex["primary_steps"] + ex["shadow_steps"],
+            "lookup_steps": lookup_steps(ex),
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": measured_traffic(mkey) if free else None,
                      "issue": measured_issue(mkey, ms) if free else None,
-                     "bytes": "lower bound: 1 B per executed step + 28 B per pixel"},
+                     "bytes": "lower bound: 1 B per executed step with a lookup + 28 B per pixel"},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -497,8 +507,7 @@ def measure_diff(args, scene):
     rays = npix + nhit
     # backward: 12 B record + 16 B target per pixel, 12 B albedo + 12 B gradient per hit
     bwd_bytes = 28 * npix + 24 * nhit
-    fwd_bytes = (ex["primary_steps"] + ex["shadow_steps"] + OUT_BYTES_PER_PIXEL * npix +
-                 12 * nhit)
+    fwd_bytes = (lookup_steps(ex) + OUT_BYTES_PER_PIXEL * npix + 12 * nhit)
     fwd_gbs = fwd_bytes / (lambert_ms * 1e-3) / 1e9
     line = {
         "metric": f"fwd+bwd ms per view ({scene}: {'1024' if batched else '256'}^3, "
@@ -535,6 +544,7 @@ def measure_diff(args, scene):
             "dda_steps": st["primary_steps"] + st["shadow_steps"],
             "free_region_test": free,
             "executed_steps": ex["primary_steps"] + ex["shadow_steps"],
+            "lookup_steps": lookup_steps(ex),
         },
         "roofline": {"bound": "hbm", "achieved": round(fwd_gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(fwd_gbs / HBM_PEAK_GBS, 5), "traffic": None},
@@ -940,6 +950,7 @@ def main():
             "gsteps_per_s": round(steps_total / (ms * 1e-3) / 1e9, 2),
             "free_region_test": free,
             "executed_steps": executed,
+            "lookup_steps": lookup_steps(ex) if args.traversal == "flat" else None,
             "tile_schedule": "blocks launched costliest first, in the order the previous "
                              "render into the frame measured (vt_frame.schedule)",
             "kernel_ms_fixed_order": round(fixed_ms, 4),

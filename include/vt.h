@@ -152,12 +152,16 @@ enum {
   VT_STAT_PRIMARY_HITS = 2,  /* = number of shadow rays traced              */
   VT_STAT_OCCLUDED = 3,      /* shadow rays that found an occluder          */
   VT_STAT_L1_FETCHES = 4,    /* bitmask traversal: 16^3 region words loaded; */
-                             /* brickmap worlds: chunk-table reads          */
+                             /* brickmap worlds: chunk-table reads; flat    */
+                             /* traversal with VT_FLAG_FREE_MAP_ON: steps   */
+                             /* taken without a lookup (a primary ray's     */
+                             /* steps before its line can reach the box of  */
+                             /* the nonempty voxels)                        */
   VT_STAT_L0_FETCHES = 5,    /* bitmask traversal: 4^3 brick words loaded;  */
                              /* brickmap worlds: block-pool + atlas reads   */
   VT_STAT_LEAF_READS = 6,    /* bitmask traversal: material bytes read at   */
                              /* primary hits (flat: 0, its gathers are one  */
-                             /* byte per executed step)                     */
+                             /* byte per executed step with a lookup)       */
   VT_STAT_ENTITY_HITS = 7,   /* shadow rays occluded by traceEntities       */
   VT_STAT_COUNT = 8
 };

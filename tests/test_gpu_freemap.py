@@ -472,3 +472,36 @@ def test_reach_box_grows_with_edits(ctx):
     assert np.array_equal(gb["hits"], o["hits"]), "reference layout, wall added: hits"
     assert np.max(np.abs(gb["shaded"] - o["shaded"])) <= 1e-5
     wb.close()
+
+
+def test_blind_steps_counted(ctx):
+    """A camera above the box of the nonempty voxels: primary rays take their
+    first steps without lookups.  The statistics of the steps executed
+    (VT_FLAG_FREE_MAP_ON) count them in the l1_fetches slot, fewer than the
+    primary steps; the images equal traceMap's (statistics of traceMap's
+    steps count none)."""
+    import torch
+    n = 128
+    g = np.zeros((n, n, n), np.uint8)
+    g[:, 0:20, :] = 3
+    g[40:60, 20:30, 50:70] = 5
+    w = vt.World.dense(ctx, g, S.palette())
+    cam = vcam.look_at((64.0, 110.0, -30.0), (64.0, 10.0, 100.0), 60.0)
+    W, H = 96, 64
+    out = {}
+    for name, flags in (("ref", 0), ("on", ON)):
+        f = poison(vt.Frame.alloc(W, H, stats=True))
+        vt.render(ctx, w, cam, f, vt.params(W, H, 1024, 1024, S.SUN, flags))
+        torch.cuda.synchronize()
+        out[name] = (f.stats_dict(), f.hits.cpu().numpy().view(np.uint32),
+                     f.shaded.cpu().numpy().view(np.uint32))
+    ref, on = out["ref"][0], out["on"][0]
+    log(f"blind steps {on['l1_fetches']} of {on['primary_steps']} primary steps executed "
+        f"(traceMap's {ref['primary_steps']})")
+    assert ref["l1_fetches"] == 0
+    assert 0 < on["l1_fetches"] < on["primary_steps"] <= ref["primary_steps"]
+    assert np.array_equal(out["ref"][1], out["on"][1])
+    assert np.array_equal(out["ref"][2], out["on"][2])
+    o = O.render(O.DenseWorld(g, S.palette()), cam, W, H, 1024, 1024, S.SUN)
+    assert np.array_equal(out["on"][1], o["hits"])
+    w.close()

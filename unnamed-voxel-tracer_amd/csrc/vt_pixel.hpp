@@ -662,6 +662,10 @@ __device__ __forceinline__ void trace_pixel_dense(const DenseWorld& w, const Vie
                                     cnt.l0f)) {
     r.blind = blind_steps(w, ex, ey, ez, pk);
     dense_run<kStats>(r, lk, w, h, cnt.l1f, cnt.l0f);
+    // statistics of the steps executed (flat, free-region test): count
+    // those taken without a lookup (VT_STAT_L1_FETCHES, unused by the flat
+    // lookups otherwise)
+    if constexpr (kStats && L::kPrefetch && L::kFree) cnt.l1f += (uint32_t)r.blind;
   }
   p.hit = h.hit;
   p.steps = h.steps;

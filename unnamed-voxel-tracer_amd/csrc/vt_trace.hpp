@@ -368,7 +368,8 @@ struct DenseRay {
                       // travels negatively stored complemented
   uint32_t flip;      // the complemented axes' masks; address = c ^ flip
   int steps, max_steps;
-  int blind;          // steps whose lookups can only read empty (blind_steps)
+  int blind;          // steps whose lookups can only read empty (blind_steps);
+                      // after dense_run_prefetch, the steps taken without them
 };
 
 // The per-ray constants of traceMap's set-up (map.glsl:85-97): the 0.001
@@ -988,10 +989,12 @@ __device__ __forceinline__ void dense_run_prefetch(DenseRay& r, L& lk, const Den
   else {
     // the lookups that can only read empty first (primary rays), the
     // iteration that reaches the cap left to the loops below
+    const int sblind = steps0;
     if (r.blind > steps0)
       dense_skip_blind<L::kSkip>(r.Px, r.Py, r.Pz, r.Ax, r.Ay, r.Az, r.dx, r.dy, r.dz, r.ix,
                                  r.iy, r.iz, mx, my, mz, flip, s0, steps0,
                                  min(r.blind, max_steps - 1));
+    r.blind = steps0 - sblind;  // the steps taken without lookups
     status = dense_skip_empty<L::kSkip, L::kFree>(
         r.Px, r.Py, r.Pz, r.Ax, r.Ay, r.Az, r.dx, r.dy, r.dz, r.ix, r.iy, r.iz, mx, my, mz, flip,
         s0, steps0, max_steps, vox, fm, free_octant_offset(flip, mx, my, mz, w.free_oct_bytes));
