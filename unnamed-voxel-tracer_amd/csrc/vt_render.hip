@@ -66,11 +66,14 @@ __attribute__((amdgpu_waves_per_eu(kShade == kShadeRef ? VT_REF_WAVES_PER_SIMD :
 // Seven waves per SIMD (at most 72 VGPRs): the engine's own frame 0.373 vs
 // 0.460 ms at eight (fewer spills around its many literal-loop iterations),
 // the C1 world in this layout 0.823 vs 0.820 ms; five waves (90 VGPRs) were
-// slower on both.
+// slower on both.  The free-region variants take six (80 VGPRs, 64 B of
+// scratch per lane against 112 at seven): with the reach box and the blind
+// prefix the C1 world in this layout measured 204.4 us at six against 212.5
+// at seven, 206.6 at five and 225.9 at eight.
 // kFree: the empty-space loops test the free-region map.
 // kSched: as k_render_dense.
 template <bool kStats, bool kFree, bool kSched>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_render_brickmap(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFree ? 6 : 7))) void k_render_brickmap(
     BrickmapWorld w, ViewConsts v, ShadeConsts sc, FrameOut out, uint32_t* sched,
     uint32_t log2_tw, uint32_t blocks_x) {
   __shared__ SchedBlock sb;
