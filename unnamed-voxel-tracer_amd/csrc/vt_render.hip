@@ -69,7 +69,8 @@ __attribute__((amdgpu_waves_per_eu(kShade == kShadeRef ? VT_REF_WAVES_PER_SIMD :
 // slower on both.  The free-region variants take six (80 VGPRs, 64 B of
 // scratch per lane against 112 at seven): with the reach box and the blind
 // prefix the C1 world in this layout measured 204.4 us at six against 212.5
-// at seven, 206.6 at five and 225.9 at eight.
+// at seven, 206.6 at five and 225.9 at eight; the engine's frame (no
+// free-region test) measures the same at six (275.7 against 275.4 us).
 // kFree: the empty-space loops test the free-region map.
 // kSched: as k_render_dense.
 template <bool kStats, bool kFree, bool kSched>
