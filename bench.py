@@ -34,12 +34,15 @@ TRAVERSALS = ["flat", "bitmask", "bitmask8"]  # VT_TRAVERSAL_FLAT, _BITMASK, _BI
 OUT_BYTES_PER_PIXEL = 16 + 12  # RGBA32F shaded + 12-byte hit record
 
 
-def lookup_steps(stats):
-    """Executed steps of the flat traversal that made their lookup: all of
-    them, less those a primary ray takes before its line can reach the box of
-    the nonempty voxels (counted in the l1_fetches slot by the statistics of
-    executed steps; 0 in traceMap's)."""
-    return stats["primary_steps"] + stats["shadow_steps"] - stats["l1_fetches"]
+def lookup_steps(stats, flat=True):
+    """Executed steps that made their lookup.  flat: statistics of the flat
+    traversal of a dense world, whose l1_fetches slot counts the steps a
+    primary ray takes before its line can reach the box of the nonempty
+    voxels (those of the steps executed; 0 in traceMap's); otherwise (the
+    bitmask traversals, the reference layout) the slot counts word fetches
+    and every step counts."""
+    steps = stats["primary_steps"] + stats["shadow_steps"]
+    return steps - stats["l1_fetches"] if flat else steps
 
 
 def algorithmic_bytes(stats, traversal, npix):
@@ -174,7 +177,8 @@ def bench_batched(args):
     ms = timed(torch, stream, batch.graph_launch, args.steps, args.warmup)
     if TRAVERSALS[best.traversal] != "flat":
         ex = st  # the bitmask traversals run every ray to its end
-    abytes = lookup_steps(ex) + OUT_BYTES_PER_PIXEL * npix
+    flat = TRAVERSALS[best.traversal] == "flat"
+    abytes = lookup_steps(ex, flat) + OUT_BYTES_PER_PIXEL * npix
     achieved = abytes / (ms * 1e-3) / 1e9
     bkey = ("k_render_batch/flat_free/1024cube_64views"
             if free and TRAVERSALS[best.traversal] == "flat" else None)
@@ -200,7 +204,7 @@ def bench_batched(args):
             "dda_steps": st["primary_steps"] + st["shadow_steps"],
             "free_region_test": free and TRAVERSALS[best.traversal] == "flat",
             "executed_steps": ex["primary_steps"] + ex["shadow_steps"],
-            "lookup_steps": lookup_steps(ex),
+            "lookup_steps": lookup_steps(ex, flat),
             "tuning": {"tile": {3: "8x8", 4: "16x4", 5: "32x2"}[best.tile],
                        "chunk": best.chunk,
                        "traversal": TRAVERSALS[best.traversal],
@@ -275,7 +279,9 @@ def bench_brickmap(args):
     # (an occupancy byte in the empty-space loop, a 4-byte chunk word in the
     # literal loop) + the outputs; the literal loop's own reads -- 4 B chunk,
     # 4 B pool, 4 B atlas per step -- are the upper figure
-    abytes = LOWER_BYTES_PER_STEP * lookup_steps(ex) + OUT_BYTES_PER_PIXEL * npix
+    # (ex: the dense world's statistics when the test is taken, else this
+    # layout's own, whose l1_fetches slot counts chunk-table reads)
+    abytes = LOWER_BYTES_PER_STEP * lookup_steps(ex, free) + OUT_BYTES_PER_PIXEL * npix
     achieved = abytes / (ms * 1e-3) / 1e9
     line = {
         "metric": "Mrays/s primary+shadow, 1080p/512³ grid in the engine's chunk/pool/atlas "
@@ -301,7 +307,7 @@ def bench_brickmap(args):
             "dda_steps": st["primary_steps"] + st["shadow_steps"],
             "free_region_test": free,
             "executed_steps": ex["primary_steps"] + ex["shadow_steps"],
-            "lookup_steps": lookup_steps(ex),
+            "lookup_steps": lookup_steps(ex, free),
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
